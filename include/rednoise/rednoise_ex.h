@@ -1,0 +1,210 @@
+/* rednoise (MI355X) -- extended C ABI.
+ *
+ * The reference ABI (rednoise.h) can only express its own fixed estimator and
+ * camera (src/capi.cpp:44-79: f = 300 px, depth 4, no Russian roulette). The
+ * entry points below carry what the physical workloads need: arrays-in scenes,
+ * an explicit camera, an FP32 HDR output, the path-replay adjoint, batched
+ * multi-scene rendering and raw ray queries. None of them has a counterpart in
+ * the reference ABI; each says which reference function it generalises.
+ *
+ * All pointers are host pointers unless the function name ends in _device, in
+ * which case they are device (HBM) pointers and `stream` is a hipStream_t
+ * (NULL = the legacy default stream). Functions return 1 on success and 0 on
+ * failure, with the reason in rn_last_error().
+ */
+#ifndef REDNOISE_EX_H
+#define REDNOISE_EX_H
+
+#include "rednoise.h"
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Thread-local description of the last failure on the calling thread ("" if
+ * none). New: the reference ABI has return codes only (src/capi.cpp:46-47). */
+const char *rn_last_error(void);
+
+/* ---------------------------------------------------------------- scenes -- */
+
+/* Scene from plain arrays: verts = 9*n floats (v0,v1,v2 per triangle),
+ * albedo = 3*n (linear, 0..1), emission = 3*n radiance or NULL (no emitters).
+ * Triangles with any emission component > 0 form the light set; the light set
+ * is fixed at creation. Generalises rn_scene_load_obj (src/capi.cpp:22-34). */
+rn_scene *rn_scene_from_arrays(const float *verts, int n, const float *albedo, const float *emission);
+
+/* Replace per-triangle albedo / emission (3*n each; either may be NULL). The
+ * light set does not change. */
+int rn_scene_set_materials(rn_scene *scene, const float *albedo, const float *emission);
+
+/* Copy the scene's triangle data out: verts 9*n, albedo 3*n, emission 3*n
+ * (any may be NULL). For an OBJ scene albedo = Kd colour int / 255. */
+int rn_scene_get_triangles(const rn_scene *scene, float *verts, float *albedo, float *emission);
+
+/* Per-triangle attributes of the reference estimator, as loadOBJ builds them
+ * (src/ObjLoader.cpp:169-211): face normal (3*n), colour ints (3*n), material
+ * id (n, enum order of framework/ModelTriangle.h:12-24), texture id (n, -1 =
+ * none), texture coordinates (6*n). Any pointer may be NULL. */
+int rn_scene_get_compat(const rn_scene *scene, float *normals, int *colour, int *material, int *texture_id,
+                        float *texcoords);
+
+/* Texture `id` of an OBJ scene: writes width/height, and if argb != NULL the
+ * width*height 0xAARRGGBB pixels (framework/TextureMap.cpp:4-31). */
+int rn_scene_get_texture(const rn_scene *scene, int id, int *width, int *height, uint32_t *argb);
+
+/* ------------------------------------------------------------ rendering -- */
+
+/* rn_render(scene, RN_PATHTRACED, ...) with an explicit 64-bit Philox key
+ * (key0, key1) for the per-pixel sample streams; rn_render uses a fixed key.
+ * Independent keys give independent renders, from which per-tile standard
+ * errors are estimated. Generalises src/capi.cpp:44-79. */
+int rn_render_seeded(const rn_scene *scene, int width, int height, float cam_z, int samples, uint32_t key0,
+                     uint32_t key1, unsigned char *rgba);
+
+/* Pinhole camera: eye, look-at target, up vector, vertical field of view. */
+typedef struct {
+	float eye[3];
+	float target[3];
+	float up[3];
+	float vfov_deg;
+} rn_camera;
+
+enum { RN_INTEGRATOR_MEGAKERNEL = 0, RN_INTEGRATOR_WAVEFRONT = 1 };
+
+typedef struct {
+	int width, height;
+	int spp;          /* samples per pixel, >= 1 */
+	int max_depth;    /* path segments traced per sample, >= 1 */
+	int rr_depth;     /* Russian roulette at vertices with depth >= rr_depth (>= max_depth: off) */
+	float rr_prob;    /* > 0: fixed survival probability; <= 0: min(0.95, max throughput) */
+	uint32_t seed;    /* Philox4x32-10 key word 0 */
+	float ray_eps;    /* origin offset along the normal; <= 0: 1e-4 * scene bounding radius */
+	int integrator;   /* RN_INTEGRATOR_* */
+	int queue_paths;  /* wavefront paths in flight; <= 0: chosen by the library */
+	int block_size;   /* threads per block for the trace kernels; <= 0: default */
+	int bvh_width;    /* 4 or 8 children per node; <= 0: 8 */
+} rn_render_params;
+
+/* Physical ("Mode P") estimator: Lambertian albedo/pi, next-event estimation to
+ * the emissive triangles (area-proportional choice), Russian roulette, black
+ * background. Writes the per-pixel mean radiance, width*height*3 FP32 (row 0 =
+ * top). Generalises renderPathTraced (src/Renderer.cpp:1119-1170), whose NEE
+ * structure follows src/Metropolis.cpp:53-81. */
+int rn_render_ex(const rn_scene *scene, const rn_camera *camera, const rn_render_params *params, float *hdr_rgb);
+
+/* As rn_render_ex with device pointers for albedo/emission (3*n, NULL = the
+ * scene's own) and the output image; enqueued on `stream`, not synchronised. */
+int rn_render_ex_device(const rn_scene *scene, const rn_camera *camera, const rn_render_params *params,
+                        const float *d_albedo, const float *d_emission, float *d_hdr_rgb, void *stream);
+
+/* Path-replay adjoint of rn_render_ex: given dL/dI (width*height*3, the loss
+ * gradient with respect to each pixel's mean radiance) writes dL/d albedo and
+ * dL/d emission (3*n each; either may be NULL). Re-traces every path from its
+ * Philox counter; no per-path state is kept between the forward pass and this
+ * call. Uses the megakernel sampling order regardless of params->integrator. */
+int rn_adjoint(const rn_scene *scene, const rn_camera *camera, const rn_render_params *params,
+               const float *dl_dimage, float *dl_dalbedo, float *dl_demission);
+
+/* Device-pointer form of rn_adjoint; gradients are accumulated into
+ * d_dl_dalbedo / d_dl_demission (the caller zeroes them). */
+int rn_adjoint_device(const rn_scene *scene, const rn_camera *camera, const rn_render_params *params,
+                      const float *d_albedo, const float *d_emission, const float *d_dl_dimage,
+                      float *d_dl_dalbedo, float *d_dl_demission, void *stream);
+
+/* One scene of a batch. */
+typedef struct {
+	const rn_scene *scene;
+	rn_camera camera;
+	rn_render_params params; /* integrator/queue/block fields are taken from the batch options */
+	float *hdr_rgb;          /* host output, width*height*3 */
+} rn_batch_item;
+
+typedef struct {
+	int queue_paths;  /* paths in flight shared by all scenes; <= 0: autotuned/default */
+	int block_size;   /* <= 0: autotuned/default */
+	int bvh_width;    /* 4 or 8; <= 0: 8 */
+	int use_graph;    /* capture the wavefront iteration in a hipGraph */
+	int iters_per_graph; /* wavefront iterations per graph launch; <= 0: 8 */
+} rn_batch_options;
+
+/* Render many scenes of uneven size in one set of persistent wavefront
+ * launches that share one path queue. Synchronous. */
+int rn_render_batch(const rn_batch_item *items, int count, const rn_batch_options *opts);
+
+/* Per-call measurements of the last rn_render_ex / rn_render_batch /
+ * rn_adjoint on the calling thread. */
+typedef struct {
+	double build_ms;        /* LBVH + BVH8 build (0 if cached) */
+	double render_ms;       /* integrator kernels, excluding the build */
+	double total_ms;        /* wall time of the call */
+	uint64_t rays_closest;  /* closest-hit queries issued */
+	uint64_t rays_shadow;   /* any-hit queries issued */
+	uint64_t paths;         /* camera paths traced */
+	double trace_kernel_ms; /* summed duration of the dominant trace kernel */
+	uint64_t trace_kernel_launches;
+	int bvh_nodes;
+	int bvh_depth;
+} rn_stats;
+
+int rn_get_last_stats(rn_stats *out);
+
+/* ---------------------------------------------------------- ray queries -- */
+
+enum { RN_QUERY_CLOSEST = 0, RN_QUERY_ANY = 1 };
+enum { RN_TRI_WATERTIGHT = 0, RN_TRI_COMPAT = 1 };
+
+/* Trace n rays (origins 3n, directions 3n, tmax n or NULL = +inf) against the
+ * scene's BVH8 on the GPU. `ignore` (n or NULL) skips one triangle per ray
+ * (src/BVH.cpp:77-111 ignoreIndex). Closest: prim (-1 = miss), t, u (weight of
+ * v1), v (weight of v2). Any: prim = 1 if occluded else 0. RN_TRI_COMPAT uses
+ * the reference triangle test (src/Geometry.cpp:11-25) in its exact FP32
+ * operation order; RN_TRI_WATERTIGHT the watertight test. */
+int rn_trace_rays(const rn_scene *scene, int n, const float *orig, const float *dir, const float *tmax,
+                  const int *ignore, int query, int tri_test, int *prim, float *t, float *u, float *v);
+
+/* Primary rays of rn_render's camera (pixel centres, no jitter,
+ * src/Camera.cpp:69-79 with f = 300) traced with the reference triangle test:
+ * per pixel prim, t, u, v and the nearest texel index ty*tw+tx of a textured
+ * hit (-1 if untextured), as selected by src/Renderer.cpp:584-591,642-656. */
+int rn_trace_primary_compat(const rn_scene *scene, int width, int height, float cam_z, int *prim, float *t,
+                            float *u, float *v, int *texel);
+
+/* -------------------------------------------------------- build / debug -- */
+
+typedef struct {
+	int nodes;        /* BVH8 nodes */
+	int depth;        /* levels, root = 1 */
+	int leaf_tris;    /* triangles referenced by leaves (== n) */
+	int width;        /* max children per node */
+	double build_ms;
+} rn_bvh_info;
+
+/* Build (or rebuild) the scene's device BVH with the given width (4 or 8). */
+int rn_scene_build(rn_scene *scene, int bvh_width, rn_bvh_info *info);
+
+/* Export the device BVH: nodes as 80-byte records (nodes*80 bytes) and the
+ * leaf-ordered primitive ids (n ints). */
+int rn_scene_export_bvh(const rn_scene *scene, void *nodes, int *leaf_prims);
+
+/* LBVH stages on the GPU for n triangles (verts 9n): 30-bit Morton codes in
+ * sorted order, the sorted primitive order, and the binary hierarchy as
+ * left/right child arrays of the n-1 internal nodes (value >= 0: internal
+ * node, value < 0: leaf ~value, i.e. sorted position -value-1). */
+int rn_lbvh_stages(const float *verts, int n, uint32_t *morton_sorted, int *order, int *left, int *right);
+
+/* Stable LSD radix sort of (key, value) pairs on the GPU over the low `bits`
+ * key bits. */
+int rn_radix_sort_pairs(uint32_t *keys, uint32_t *values, int n, int bits);
+
+/* Philox4x32-10 on the GPU: out[4i..4i+3] = philox(ctr[4i..4i+3], key[2i..2i+1]). */
+int rn_philox4x32_10(int n, const uint32_t *ctr, const uint32_t *key, uint32_t *out);
+
+/* Name of the GPU the library runs on (e.g. "gfx950"), or "" if none. */
+const char *rn_device_name(void);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+#endif /* REDNOISE_EX_H */

@@ -1,0 +1,248 @@
+"""ctypes wrapper of the CPU oracle (liboracle.so) and, when built, of the
+compiled reference (oracle/_ref/librednoise_ref.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py -- never by the rednoise_amd package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+REF_PATH = os.path.join(HERE, "_ref", "librednoise_ref.so")
+
+_lib = None
+_ref = None
+
+f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+
+
+class Cam(C.Structure):
+    _fields_ = [("eye", C.c_float * 3), ("du", C.c_float * 3), ("dv", C.c_float * 3), ("w", C.c_float * 3)]
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("width", C.c_int), ("height", C.c_int), ("spp", C.c_int), ("max_depth", C.c_int), ("rr_depth", C.c_int),
+        ("rr_prob", C.c_float), ("seed", C.c_uint32), ("scene_id", C.c_uint32), ("ray_eps", C.c_float),
+    ]
+
+
+class Counts(C.Structure):
+    _fields_ = [("closest", C.c_uint64), ("shadow", C.c_uint64), ("paths", C.c_uint64)]
+
+
+class CompatScene(C.Structure):
+    _fields_ = [
+        ("n", C.c_int), ("verts", C.c_void_p), ("normals", C.c_void_p), ("colour", C.c_void_p),
+        ("material", C.c_void_p), ("texture_id", C.c_void_p), ("texcoords", C.c_void_p),
+        ("n_textures", C.c_int), ("tex_w", C.c_void_p), ("tex_h", C.c_void_p), ("tex", C.c_void_p),
+    ]
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.isfile(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        L.or_philox4x32_10.argtypes = [u32p, u32p, u32p]
+        L.or_philox_batch.argtypes = [C.c_int, u32p, u32p, u32p]
+        L.or_compat_intersect_tri.argtypes = [f32p] * 5 + [C.POINTER(C.c_float)] * 3
+        L.or_compat_intersect_tri.restype = C.c_int
+        L.or_compat_primary_ray.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float,
+                                            f32p, f32p]
+        L.or_compat_closest_brute.argtypes = [f32p, C.c_int, f32p, f32p, C.c_int] + [C.POINTER(C.c_float)] * 3
+        L.or_compat_closest_brute.restype = C.c_int
+        L.or_compat_bvh_build.argtypes = [f32p, C.c_int]
+        L.or_compat_bvh_build.restype = C.c_void_p
+        L.or_compat_bvh_free.argtypes = [C.c_void_p]
+        L.or_compat_bvh_intersect_batch.argtypes = [C.c_void_p, C.c_int, f32p, f32p, C.c_void_p, i32p, f32p, f32p,
+                                                    f32p]
+        L.or_compat_bvh_occluded.argtypes = [C.c_void_p, f32p, f32p, C.c_float, C.c_int]
+        L.or_compat_bvh_occluded.restype = C.c_int
+        L.or_compat_texel.argtypes = [C.c_float, C.c_float, f32p, C.c_int, C.c_int]
+        L.or_compat_texel.restype = C.c_int
+        L.or_perlin.argtypes = [C.c_float] * 3
+        L.or_perlin.restype = C.c_float
+        L.or_fractal_noise.argtypes = [f32p, C.c_int]
+        L.or_fractal_noise.restype = C.c_float
+        L.or_compat_render.argtypes = [C.POINTER(CompatScene), C.c_int, C.c_int, C.c_float, C.c_int, C.c_uint32,
+                                       C.c_uint32, C.c_int, C.c_int, u8p]
+        L.or_morton3.argtypes = [C.c_uint32] * 3
+        L.or_morton3.restype = C.c_uint32
+        L.or_morton_codes.argtypes = [f32p, C.c_int, u32p]
+        L.or_radix_tree.argtypes = [u32p, C.c_int, i32p, i32p]
+        L.or_camera_basis.argtypes = [f32p, f32p, f32p, C.c_float, C.c_int, C.c_int, C.POINTER(Cam)]
+        for suf, fp in (("f32", f32p), ("f64", f64p)):
+            getattr(L, f"or_phys_prepare_{suf}").argtypes = [f32p, C.c_int, f32p, C.c_void_p]
+            getattr(L, f"or_phys_prepare_{suf}").restype = C.c_void_p
+            getattr(L, f"or_phys_free_{suf}").argtypes = [C.c_void_p]
+            getattr(L, f"or_phys_render_{suf}").argtypes = [C.c_void_p, C.POINTER(Cam), C.POINTER(Params), C.c_int,
+                                                           C.c_int, C.c_int, fp, C.POINTER(Counts)]
+            getattr(L, f"or_phys_render_list_{suf}").argtypes = [C.c_void_p, C.POINTER(Cam), C.POINTER(Params),
+                                                                C.c_int, i32p, C.c_int, fp, C.POINTER(Counts)]
+        L.or_phys_adjoint_f64.argtypes = [C.c_void_p, C.POINTER(Cam), C.POINTER(Params), f32p, C.c_int, f64p, f64p]
+        L.or_phys_closest_f64.argtypes = [C.c_void_p, f64p, f64p, C.c_double, C.c_int] + [C.POINTER(C.c_double)] * 3
+        L.or_phys_closest_f64.restype = C.c_int
+        L.or_phys_closest_f32.argtypes = [C.c_void_p, f32p, f32p, C.c_float, C.c_int] + [C.POINTER(C.c_float)] * 3
+        L.or_phys_closest_f32.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def ref():
+    """The compiled reference, or None when it has not been built."""
+    global _ref
+    if _ref is None and os.path.isfile(REF_PATH):
+        L = C.CDLL(REF_PATH)
+        L.ref_load_obj.argtypes = [C.c_char_p, C.c_float] + [C.c_void_p] * 6
+        L.ref_load_obj.restype = C.c_int
+        L.ref_bvh_intersect.argtypes = [f32p, C.c_int, C.c_int, f32p, f32p, C.c_void_p, i32p, f32p, f32p, f32p]
+        L.ref_bvh_occluded.argtypes = [f32p, C.c_int, C.c_int, f32p, f32p, f32p, C.c_void_p, i32p]
+        L.ref_brute_closest.argtypes = [f32p, C.c_int, C.c_int, f32p, f32p, i32p, f32p, f32p, f32p]
+        L.ref_intersect_triangle.argtypes = [C.c_int, f32p, f32p, f32p, i32p, f32p, f32p, f32p]
+        L.ref_primary_rays.argtypes = [C.c_int, C.c_int, C.c_float, C.c_void_p, f32p, f32p]
+        L.ref_fractal_noise.argtypes = [C.c_int, f32p, C.c_int, f32p]
+        L.rn_scene_load_obj.argtypes = [C.c_char_p, C.c_float]
+        L.rn_scene_load_obj.restype = C.c_void_p
+        L.rn_scene_free.argtypes = [C.c_void_p]
+        L.rn_render.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int, u8p]
+        L.rn_render.restype = C.c_int
+        _ref = L
+    return _ref
+
+
+# ----------------------------------------------------------------- helpers --
+
+def philox(ctr: np.ndarray, key: np.ndarray) -> np.ndarray:
+    ctr = np.ascontiguousarray(ctr, dtype=np.uint32).reshape(-1, 4)
+    key = np.ascontiguousarray(key, dtype=np.uint32).reshape(-1, 2)
+    out = np.empty_like(ctr)
+    lib().or_philox_batch(ctr.shape[0], ctr, key, out)
+    return out
+
+
+def compat_bvh_intersect(verts: np.ndarray, o: np.ndarray, d: np.ndarray, ignore=None):
+    L = lib()
+    verts = np.ascontiguousarray(verts, dtype=np.float32).reshape(-1)
+    o = np.ascontiguousarray(o, dtype=np.float32).reshape(-1)
+    d = np.ascontiguousarray(d, dtype=np.float32).reshape(-1)
+    nr = o.size // 3
+    h = L.or_compat_bvh_build(verts, verts.size // 9)
+    prim = np.empty(nr, np.int32)
+    t = np.empty(nr, np.float32)
+    u = np.empty(nr, np.float32)
+    v = np.empty(nr, np.float32)
+    ig = None
+    if ignore is not None:
+        ig_arr = np.ascontiguousarray(ignore, dtype=np.int32)
+        ig = ig_arr.ctypes.data
+    L.or_compat_bvh_intersect_batch(h, nr, o, d, ig, prim, t, u, v)
+    L.or_compat_bvh_free(h)
+    return prim, t, u, v
+
+
+def compat_brute(verts, o, d):
+    L = lib()
+    verts = np.ascontiguousarray(verts, dtype=np.float32).reshape(-1)
+    o = np.ascontiguousarray(o, dtype=np.float32).reshape(-1, 3)
+    d = np.ascontiguousarray(d, dtype=np.float32).reshape(-1, 3)
+    n = verts.size // 9
+    prim = np.empty(len(o), np.int32)
+    t = np.empty(len(o), np.float32)
+    u = np.empty(len(o), np.float32)
+    v = np.empty(len(o), np.float32)
+    for i in range(len(o)):
+        tt, uu, vv = C.c_float(), C.c_float(), C.c_float()
+        prim[i] = L.or_compat_closest_brute(verts, n, np.ascontiguousarray(o[i]), np.ascontiguousarray(d[i]), -1,
+                                            C.byref(tt), C.byref(uu), C.byref(vv))
+        t[i], u[i], v[i] = tt.value, uu.value, vv.value
+    return prim, t, u, v
+
+
+def compat_primary_rays(w: int, h: int, cam_z: float):
+    L = lib()
+    o = np.empty((h * w, 3), np.float32)
+    d = np.empty((h * w, 3), np.float32)
+    oo = np.empty(3, np.float32)
+    dd = np.empty(3, np.float32)
+    for y in range(h):
+        for x in range(w):
+            L.or_compat_primary_ray(x, y, w, h, cam_z, 0.0, 0.0, oo, dd)
+            o[y * w + x] = oo
+            d[y * w + x] = dd
+    return o, d
+
+
+def camera_basis(eye, target, up, vfov, w, h) -> Cam:
+    cam = Cam()
+    lib().or_camera_basis(np.asarray(eye, np.float32), np.asarray(target, np.float32), np.asarray(up, np.float32),
+                          float(vfov), int(w), int(h), C.byref(cam))
+    return cam
+
+
+def params(width, height, spp, max_depth, rr_depth, rr_prob=0.0, seed=0, scene_id=0, ray_eps=1e-3) -> Params:
+    return Params(int(width), int(height), int(spp), int(max_depth), int(rr_depth), float(rr_prob), int(seed),
+                  int(scene_id), float(ray_eps))
+
+
+class PhysScene:
+    """Prepared physical-estimator scene in FP32 or FP64."""
+
+    def __init__(self, verts, albedo, emission=None, precision="f64"):
+        self.L = lib()
+        self.suf = precision
+        self.verts = np.ascontiguousarray(verts, dtype=np.float32).reshape(-1)
+        self.albedo = np.ascontiguousarray(albedo, dtype=np.float32).reshape(-1)
+        self.n = self.verts.size // 9
+        self.emission = None if emission is None else np.ascontiguousarray(emission, dtype=np.float32).reshape(-1)
+        em = None if self.emission is None else self.emission.ctypes.data
+        self.h = getattr(self.L, f"or_phys_prepare_{precision}")(self.verts, self.n, self.albedo, em)
+
+    def close(self):
+        if self.h:
+            getattr(self.L, f"or_phys_free_{self.suf}")(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def render(self, cam: Cam, p: Params, threads: int = 0, pixels=None):
+        dt = np.float64 if self.suf == "f64" else np.float32
+        cnt = Counts()
+        threads = threads or os.cpu_count() or 1
+        if pixels is None:
+            out = np.zeros(p.width * p.height * 3, dt)
+            getattr(self.L, f"or_phys_render_{self.suf}")(self.h, C.byref(cam), C.byref(p), 0, p.width * p.height,
+                                                          threads, out, C.byref(cnt))
+            return out.reshape(p.height, p.width, 3), cnt
+        pix = np.ascontiguousarray(pixels, dtype=np.int32)
+        out = np.zeros(pix.size * 3, dt)
+        getattr(self.L, f"or_phys_render_list_{self.suf}")(self.h, C.byref(cam), C.byref(p), pix.size, pix, threads,
+                                                           out, C.byref(cnt))
+        return out.reshape(-1, 3), cnt
+
+    def adjoint(self, cam: Cam, p: Params, dl_dimage, threads: int = 0):
+        assert self.suf == "f64"
+        ga = np.zeros(3 * self.n, np.float64)
+        ge = np.zeros(3 * self.n, np.float64)
+        g = np.ascontiguousarray(dl_dimage, dtype=np.float32).reshape(-1)
+        self.L.or_phys_adjoint_f64(self.h, C.byref(cam), C.byref(p), g, threads or os.cpu_count() or 1, ga, ge)
+        return ga.reshape(-1, 3), ge.reshape(-1, 3)

@@ -1,0 +1,505 @@
+"""rednoise_amd: MI355X-native path tracer behind the rednoise C ABI.
+
+The public surface mirrors the reference Python binding
+(bindings/python/rednoise/__init__.py: ``RenderMode``, ``Scene.load_obj``,
+``Scene.render``, ``Scene.triangle_count``, ``Scene.close``, ``Image``,
+``version``) over ``librednoise.so``, which this package builds in place
+(``rednoise_amd/librednoise.so``). Rendering runs on the GPU only: there is no
+CPU fallback, and every call raises when the library or the GPU is missing.
+
+Additions beyond the reference binding (the extended ABI, rednoise_ex.h):
+``Scene.from_arrays``, ``Scene.render_hdr`` (physical estimator),
+``Scene.adjoint``, ``Scene.trace_rays``, ``Scene.trace_primary_compat``,
+``Scene.build``, ``render_batch`` and ``last_stats``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from enum import IntEnum
+from typing import Optional, Sequence, Tuple, Union
+
+import numpy as np
+
+__all__ = [
+    "RenderMode", "Scene", "Image", "version", "Camera", "RenderParams", "BatchOptions", "render_batch",
+    "last_stats", "library_path", "device_name", "INTEGRATOR_MEGAKERNEL", "INTEGRATOR_WAVEFRONT",
+]
+
+__version__ = "0.1.2"
+
+INTEGRATOR_MEGAKERNEL = 0
+INTEGRATOR_WAVEFRONT = 1
+QUERY_CLOSEST = 0
+QUERY_ANY = 1
+TRI_WATERTIGHT = 0
+TRI_COMPAT = 1
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+class RenderMode(IntEnum):
+    """Which renderer to run (the C ``rn_render_mode`` enum)."""
+
+    WIREFRAME = 0
+    RASTERISED = 1
+    RAYTRACED = 2
+    PATHTRACED = 3
+
+    @classmethod
+    def coerce(cls, value: Union["RenderMode", int, str]) -> "RenderMode":
+        """Coerce an int, a case-insensitive name or a member
+        (bindings/python/rednoise/__init__.py:52-74 semantics)."""
+        if isinstance(value, cls):
+            return value
+        if isinstance(value, bool):
+            raise TypeError("render mode must not be a bool")
+        if isinstance(value, int):
+            return cls(value)
+        if isinstance(value, str):
+            key = value.strip().upper().replace("-", "").replace("_", "").replace(" ", "")
+            for member in cls:
+                if member.name.replace("_", "") == key:
+                    return member
+            valid = ", ".join(m.name.lower() for m in cls)
+            raise ValueError(f"unknown render mode {value!r}; expected one of: {valid}")
+        raise TypeError(f"render mode must be RenderMode, int or str, not {type(value).__name__}")
+
+
+# ----------------------------------------------------------------- structs --
+
+class Camera(C.Structure):
+    """rn_camera: eye, look-at target, up, vertical field of view (degrees)."""
+
+    _fields_ = [("eye", C.c_float * 3), ("target", C.c_float * 3), ("up", C.c_float * 3), ("vfov_deg", C.c_float)]
+
+    @classmethod
+    def make(cls, eye, target, up=(0.0, 1.0, 0.0), vfov_deg=40.0) -> "Camera":
+        return cls((C.c_float * 3)(*eye), (C.c_float * 3)(*target), (C.c_float * 3)(*up), float(vfov_deg))
+
+
+class RenderParams(C.Structure):
+    """rn_render_params."""
+
+    _fields_ = [
+        ("width", C.c_int), ("height", C.c_int), ("spp", C.c_int), ("max_depth", C.c_int), ("rr_depth", C.c_int),
+        ("rr_prob", C.c_float), ("seed", C.c_uint32), ("ray_eps", C.c_float), ("integrator", C.c_int),
+        ("queue_paths", C.c_int), ("block_size", C.c_int), ("bvh_width", C.c_int),
+    ]
+
+    @classmethod
+    def make(cls, width, height, spp, max_depth, rr_depth=1 << 30, rr_prob=0.0, seed=0, ray_eps=0.0,
+             integrator=INTEGRATOR_MEGAKERNEL, queue_paths=0, block_size=0, bvh_width=8) -> "RenderParams":
+        return cls(int(width), int(height), int(spp), int(max_depth), int(min(rr_depth, 1 << 30)), float(rr_prob),
+                   int(seed) & 0xFFFFFFFF, float(ray_eps), int(integrator), int(queue_paths), int(block_size),
+                   int(bvh_width))
+
+
+class BatchItem(C.Structure):
+    _fields_ = [("scene", C.c_void_p), ("camera", Camera), ("params", RenderParams), ("hdr_rgb", C.c_void_p)]
+
+
+class BatchOptions(C.Structure):
+    """rn_batch_options."""
+
+    _fields_ = [("queue_paths", C.c_int), ("block_size", C.c_int), ("bvh_width", C.c_int), ("use_graph", C.c_int),
+                ("iters_per_graph", C.c_int)]
+
+
+class Stats(C.Structure):
+    """rn_stats of the last call on this thread."""
+
+    _fields_ = [
+        ("build_ms", C.c_double), ("render_ms", C.c_double), ("total_ms", C.c_double),
+        ("rays_closest", C.c_uint64), ("rays_shadow", C.c_uint64), ("paths", C.c_uint64),
+        ("trace_kernel_ms", C.c_double), ("trace_kernel_launches", C.c_uint64), ("bvh_nodes", C.c_int),
+        ("bvh_depth", C.c_int),
+    ]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class BvhInfo(C.Structure):
+    _fields_ = [("nodes", C.c_int), ("depth", C.c_int), ("leaf_tris", C.c_int), ("width", C.c_int),
+                ("build_ms", C.c_double)]
+
+
+# ------------------------------------------------------------ the library --
+
+_LIB = None
+
+
+def library_path() -> str:
+    """Path of the in-tree librednoise.so (REDNOISE_LIBRARY overrides)."""
+    env = os.environ.get("REDNOISE_LIBRARY")
+    if env:
+        if not os.path.isfile(env):
+            raise OSError(f"REDNOISE_LIBRARY is set to {env!r} but that file does not exist.")
+        return env
+    return os.path.join(_HERE, "librednoise.so")
+
+
+def _load_library() -> C.CDLL:
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    path = library_path()
+    if not os.path.isfile(path):
+        raise OSError(f"librednoise.so not found at {path!r}; build it with `python -c \"import __graft_entry__ as g; "
+                      "g.build()\"` or `make -C rednoise_amd/csrc`.")
+    lib = C.CDLL(path)
+    _configure(lib)
+    _LIB = lib
+    return lib
+
+
+def _configure(lib: C.CDLL) -> None:
+    vp, i, f, cp = C.c_void_p, C.c_int, C.c_float, C.c_char_p
+    sig = {
+        "rn_scene_load_obj": ([cp, f], vp), "rn_scene_free": ([vp], None), "rn_scene_triangle_count": ([vp], i),
+        "rn_render": ([vp, i, i, i, f, i, vp], i), "rn_render_seeded": ([vp, i, i, f, i, C.c_uint32, C.c_uint32, vp], i), "rn_save_png": ([cp, i, i, vp], i), "rn_version": ([], cp),
+        "rn_last_error": ([], cp), "rn_device_name": ([], cp),
+        "rn_scene_from_arrays": ([vp, i, vp, vp], vp), "rn_scene_set_materials": ([vp, vp, vp], i),
+        "rn_scene_get_triangles": ([vp, vp, vp, vp], i), "rn_scene_get_compat": ([vp, vp, vp, vp, vp, vp], i),
+        "rn_scene_get_texture": ([vp, i, vp, vp, vp], i),
+        "rn_render_ex": ([vp, C.POINTER(Camera), C.POINTER(RenderParams), vp], i),
+        "rn_render_ex_device": ([vp, C.POINTER(Camera), C.POINTER(RenderParams), vp, vp, vp, vp], i),
+        "rn_adjoint": ([vp, C.POINTER(Camera), C.POINTER(RenderParams), vp, vp, vp], i),
+        "rn_adjoint_device": ([vp, C.POINTER(Camera), C.POINTER(RenderParams), vp, vp, vp, vp, vp, vp], i),
+        "rn_render_batch": ([C.POINTER(BatchItem), i, C.POINTER(BatchOptions)], i),
+        "rn_get_last_stats": ([C.POINTER(Stats)], i),
+        "rn_trace_rays": ([vp, i, vp, vp, vp, vp, i, i, vp, vp, vp, vp], i),
+        "rn_trace_primary_compat": ([vp, i, i, f, vp, vp, vp, vp, vp], i),
+        "rn_scene_build": ([vp, i, C.POINTER(BvhInfo)], i), "rn_scene_export_bvh": ([vp, vp, vp], i),
+        "rn_lbvh_stages": ([vp, i, vp, vp, vp, vp], i), "rn_radix_sort_pairs": ([vp, vp, i, i], i),
+        "rn_philox4x32_10": ([i, vp, vp, vp], i),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+
+
+def _ptr(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _err(what: str) -> RuntimeError:
+    msg = _load_library().rn_last_error().decode(errors="replace")
+    return RuntimeError(f"{what} failed: {msg}")
+
+
+def version() -> str:
+    """Library version string (``rn_version``)."""
+    return _load_library().rn_version().decode()
+
+
+def device_name() -> str:
+    return _load_library().rn_device_name().decode()
+
+
+def last_stats() -> dict:
+    s = Stats()
+    _load_library().rn_get_last_stats(C.byref(s))
+    return s.as_dict()
+
+
+def _f32(a, n=None) -> np.ndarray:
+    a = np.ascontiguousarray(a, dtype=np.float32).reshape(-1)
+    if n is not None and a.size != n:
+        raise ValueError(f"expected {n} floats, got {a.size}")
+    return a
+
+
+# ------------------------------------------------------------------ image --
+
+class Image:
+    """An RGBA8 image (bindings/python/rednoise/__init__.py:240-325)."""
+
+    __slots__ = ("_rgba", "_width", "_height")
+
+    def __init__(self, rgba: bytes, width: int, height: int):
+        expected = width * height * 4
+        if len(rgba) != expected:
+            raise ValueError(f"rgba buffer has {len(rgba)} bytes, expected {expected} for a {width}x{height} "
+                             "RGBA8 image")
+        self._rgba = bytes(rgba)
+        self._width = int(width)
+        self._height = int(height)
+
+    @property
+    def rgba(self) -> bytes:
+        return self._rgba
+
+    @property
+    def width(self) -> int:
+        return self._width
+
+    @property
+    def height(self) -> int:
+        return self._height
+
+    @property
+    def size(self) -> Tuple[int, int]:
+        return (self._width, self._height)
+
+    def to_numpy(self) -> np.ndarray:
+        return np.frombuffer(self._rgba, dtype=np.uint8).reshape(self._height, self._width, 4)
+
+    def save(self, path: Union[str, os.PathLike]) -> None:
+        """Write a PNG through ``rn_save_png``."""
+        path = os.fspath(path)
+        ext = os.path.splitext(path)[1].lower()
+        if ext and ext != ".png":
+            raise ValueError(f"only .png is supported; got {ext!r}")
+        lib = _load_library()
+        buf = np.frombuffer(self._rgba, dtype=np.uint8)
+        if lib.rn_save_png(os.fsencode(path), self._width, self._height, _ptr(buf)) != 1:
+            raise RuntimeError(f"rn_save_png failed to write {path!r}")
+
+    def __repr__(self) -> str:
+        return f"<rednoise.Image {self._width}x{self._height} RGBA8>"
+
+
+# ------------------------------------------------------------------ scene --
+
+class Scene:
+    """A scene backed by a native ``rn_scene`` handle."""
+
+    __slots__ = ("_handle", "_n", "__weakref__")
+
+    def __init__(self, handle: int):
+        if not handle:
+            raise ValueError("Scene requires a non-NULL native handle")
+        self._handle = handle
+        self._n = int(_load_library().rn_scene_triangle_count(handle))
+
+    @classmethod
+    def load_obj(cls, path: Union[str, os.PathLike], scale: float = 0.35) -> "Scene":
+        """Load a Wavefront OBJ (+ .mtl). Raises FileNotFoundError / RuntimeError."""
+        path = os.fspath(path)
+        if not os.path.isfile(path):
+            raise FileNotFoundError(f"OBJ file not found: {path!r}")
+        lib = _load_library()
+        h = lib.rn_scene_load_obj(os.fsencode(path), C.c_float(scale))
+        if not h:
+            raise _err(f"rn_scene_load_obj({path!r})")
+        return cls(h)
+
+    @classmethod
+    def from_arrays(cls, verts, albedo, emission=None) -> "Scene":
+        """Scene from (n,3,3) vertices, (n,3) albedo and optional (n,3) emission."""
+        v = _f32(verts)
+        n = v.size // 9
+        a = _f32(albedo, 3 * n)
+        e = None if emission is None else _f32(emission, 3 * n)
+        h = _load_library().rn_scene_from_arrays(_ptr(v), n, _ptr(a), _ptr(e))
+        if not h:
+            raise _err("rn_scene_from_arrays")
+        return cls(h)
+
+    @property
+    def handle(self) -> int:
+        self._check_open()
+        return self._handle
+
+    @property
+    def triangle_count(self) -> int:
+        self._check_open()
+        return int(_load_library().rn_scene_triangle_count(self._handle))
+
+    def render(self, mode: Union[RenderMode, int, str] = "pathtraced", width: int = 640, height: int = 480,
+               cam_z: float = 4.0, samples: int = 64) -> Image:
+        """``rn_render``: RGBA8 image of the fixed (0,0,cam_z) camera."""
+        self._check_open()
+        m = RenderMode.coerce(mode)
+        width, height = int(width), int(height)
+        if width <= 0 or height <= 0:
+            raise ValueError(f"width and height must be positive, got {width}x{height}")
+        buf = np.empty(width * height * 4, np.uint8)
+        ok = _load_library().rn_render(self._handle, int(m), width, height, C.c_float(cam_z), int(samples), _ptr(buf))
+        if ok != 1:
+            raise _err(f"rn_render(mode={m.name}, {width}x{height})")
+        return Image(buf.tobytes(), width, height)
+
+    def render_seeded(self, width: int, height: int, cam_z: float, samples: int, key0: int, key1: int = 0) -> Image:
+        """``rn_render_seeded``: the reference estimator with an explicit Philox key."""
+        self._check_open()
+        buf = np.empty(width * height * 4, np.uint8)
+        if _load_library().rn_render_seeded(self._handle, width, height, C.c_float(cam_z), int(samples),
+                                            int(key0) & 0xFFFFFFFF, int(key1) & 0xFFFFFFFF, _ptr(buf)) != 1:
+            raise _err("rn_render_seeded")
+        return Image(buf.tobytes(), width, height)
+
+    def render_hdr(self, camera: Camera, params: RenderParams) -> np.ndarray:
+        """``rn_render_ex``: (H, W, 3) float32 mean radiance."""
+        self._check_open()
+        out = np.empty(params.width * params.height * 3, np.float32)
+        if _load_library().rn_render_ex(self._handle, C.byref(camera), C.byref(params), _ptr(out)) != 1:
+            raise _err("rn_render_ex")
+        return out.reshape(params.height, params.width, 3)
+
+    def adjoint(self, camera: Camera, params: RenderParams, dl_dimage) -> Tuple[np.ndarray, np.ndarray]:
+        """``rn_adjoint``: (dL/d albedo, dL/d emission), each (n, 3)."""
+        self._check_open()
+        g = _f32(dl_dimage, params.width * params.height * 3)
+        ga = np.empty(3 * self._n, np.float32)
+        ge = np.empty(3 * self._n, np.float32)
+        if _load_library().rn_adjoint(self._handle, C.byref(camera), C.byref(params), _ptr(g), _ptr(ga),
+                                      _ptr(ge)) != 1:
+            raise _err("rn_adjoint")
+        return ga.reshape(-1, 3), ge.reshape(-1, 3)
+
+    def set_materials(self, albedo=None, emission=None) -> None:
+        a = None if albedo is None else _f32(albedo, 3 * self._n)
+        e = None if emission is None else _f32(emission, 3 * self._n)
+        if _load_library().rn_scene_set_materials(self._handle, _ptr(a), _ptr(e)) != 1:
+            raise _err("rn_scene_set_materials")
+
+    def triangles(self) -> Tuple[np.ndarray, np.ndarray, np.ndarray]:
+        v = np.empty(9 * self._n, np.float32)
+        a = np.empty(3 * self._n, np.float32)
+        e = np.empty(3 * self._n, np.float32)
+        _load_library().rn_scene_get_triangles(self._handle, _ptr(v), _ptr(a), _ptr(e))
+        return v.reshape(-1, 3, 3), a.reshape(-1, 3), e.reshape(-1, 3)
+
+    def compat_attributes(self) -> dict:
+        n = self._n
+        nm = np.empty(3 * n, np.float32)
+        col = np.empty(3 * n, np.int32)
+        mat = np.empty(n, np.int32)
+        tid = np.empty(n, np.int32)
+        tc = np.empty(6 * n, np.float32)
+        _load_library().rn_scene_get_compat(self._handle, _ptr(nm), _ptr(col), _ptr(mat), _ptr(tid), _ptr(tc))
+        return {"normals": nm.reshape(-1, 3), "colour": col.reshape(-1, 3), "material": mat, "texture_id": tid,
+                "texcoords": tc.reshape(-1, 6)}
+
+    def texture(self, tex_id: int) -> np.ndarray:
+        w, h = C.c_int(), C.c_int()
+        if _load_library().rn_scene_get_texture(self._handle, tex_id, C.byref(w), C.byref(h), None) != 1:
+            raise _err("rn_scene_get_texture")
+        px = np.empty(w.value * h.value, np.uint32)
+        _load_library().rn_scene_get_texture(self._handle, tex_id, C.byref(w), C.byref(h), _ptr(px))
+        return px.reshape(h.value, w.value)
+
+    def build(self, bvh_width: int = 8) -> dict:
+        info = BvhInfo()
+        if _load_library().rn_scene_build(self._handle, int(bvh_width), C.byref(info)) != 1:
+            raise _err("rn_scene_build")
+        return {k: getattr(info, k) for k, _ in BvhInfo._fields_}
+
+    def export_bvh(self, bvh_width: int = 8) -> Tuple[np.ndarray, np.ndarray]:
+        """Rebuild the BVH and return (nodes as (m, 80) bytes, leaf-ordered prim ids)."""
+        info = self.build(bvh_width)
+        nodes = np.empty(info["nodes"] * 80, np.uint8)
+        prims = np.empty(self._n, np.int32)
+        if _load_library().rn_scene_export_bvh(self._handle, _ptr(nodes), _ptr(prims)) != 1:
+            raise _err("rn_scene_export_bvh")
+        return nodes.reshape(-1, 80), prims
+
+    def trace_rays(self, origins, dirs, tmax=None, ignore=None, query=QUERY_CLOSEST, tri_test=TRI_WATERTIGHT):
+        """``rn_trace_rays``: returns (prim, t, u, v) for closest, occluded flags for any."""
+        self._check_open()
+        o = _f32(origins)
+        d = _f32(dirs, o.size)
+        n = o.size // 3
+        tm = None if tmax is None else _f32(tmax, n)
+        ig = None if ignore is None else np.ascontiguousarray(ignore, dtype=np.int32)
+        prim = np.empty(n, np.int32)
+        t = np.empty(n, np.float32)
+        u = np.empty(n, np.float32)
+        v = np.empty(n, np.float32)
+        if _load_library().rn_trace_rays(self._handle, n, _ptr(o), _ptr(d), _ptr(tm), _ptr(ig), int(query),
+                                         int(tri_test), _ptr(prim), _ptr(t), _ptr(u), _ptr(v)) != 1:
+            raise _err("rn_trace_rays")
+        if query == QUERY_ANY:
+            return prim
+        return prim, t, u, v
+
+    def trace_primary_compat(self, width: int, height: int, cam_z: float = 4.0):
+        n = width * height
+        prim = np.empty(n, np.int32)
+        t = np.empty(n, np.float32)
+        u = np.empty(n, np.float32)
+        v = np.empty(n, np.float32)
+        tx = np.empty(n, np.int32)
+        if _load_library().rn_trace_primary_compat(self._handle, width, height, C.c_float(cam_z), _ptr(prim), _ptr(t),
+                                                   _ptr(u), _ptr(v), _ptr(tx)) != 1:
+            raise _err("rn_trace_primary_compat")
+        return prim, t, u, v, tx
+
+    def close(self) -> None:
+        """Release the native scene; safe to call more than once."""
+        h = self._handle
+        if h:
+            self._handle = 0
+            try:
+                if _LIB is not None:
+                    _LIB.rn_scene_free(h)
+            except Exception:  # pragma: no cover - interpreter shutdown
+                pass
+
+    def _check_open(self) -> None:
+        if not self._handle:
+            raise ValueError("scene is closed")
+
+    def __enter__(self) -> "Scene":
+        return self
+
+    def __exit__(self, *exc) -> None:
+        self.close()
+
+    def __del__(self):
+        self.close()
+
+    def __repr__(self) -> str:
+        return f"<rednoise.Scene {self._n} triangles>" if self._handle else "<rednoise.Scene (closed)>"
+
+
+def render_batch(items: Sequence[Tuple[Scene, Camera, RenderParams]], options: Optional[BatchOptions] = None):
+    """``rn_render_batch``: list of (H, W, 3) images, one per item."""
+    lib = _load_library()
+    outs = []
+    arr = (BatchItem * len(items))()
+    for k, (sc, cam, p) in enumerate(items):
+        o = np.empty(p.width * p.height * 3, np.float32)
+        outs.append(o)
+        arr[k].scene = sc.handle
+        arr[k].camera = cam
+        arr[k].params = p
+        arr[k].hdr_rgb = o.ctypes.data
+    opts = options if options is not None else BatchOptions(0, 0, 8, 0, 0)
+    if lib.rn_render_batch(arr, len(items), C.byref(opts)) != 1:
+        raise _err("rn_render_batch")
+    return [o.reshape(p.height, p.width, 3) for o, (_, _, p) in zip(outs, items)]
+
+
+def lbvh_stages(verts):
+    v = _f32(verts)
+    n = v.size // 9
+    codes = np.empty(n, np.uint32)
+    order = np.empty(n, np.int32)
+    left = np.empty(max(n - 1, 1), np.int32)
+    right = np.empty(max(n - 1, 1), np.int32)
+    if _load_library().rn_lbvh_stages(_ptr(v), n, _ptr(codes), _ptr(order), _ptr(left), _ptr(right)) != 1:
+        raise _err("rn_lbvh_stages")
+    return codes, order, left[: n - 1], right[: n - 1]
+
+
+def radix_sort_pairs(keys, values, bits=32):
+    k = np.ascontiguousarray(keys, dtype=np.uint32).copy()
+    v = np.ascontiguousarray(values, dtype=np.uint32).copy()
+    if _load_library().rn_radix_sort_pairs(_ptr(k), _ptr(v), k.size, int(bits)) != 1:
+        raise _err("rn_radix_sort_pairs")
+    return k, v
+
+
+def philox(ctr, key):
+    c = np.ascontiguousarray(ctr, dtype=np.uint32).reshape(-1)
+    k = np.ascontiguousarray(key, dtype=np.uint32).reshape(-1)
+    n = c.size // 4
+    out = np.empty(4 * n, np.uint32)
+    if _load_library().rn_philox4x32_10(n, _ptr(c), _ptr(k), _ptr(out)) != 1:
+        raise _err("rn_philox4x32_10")
+    return out.reshape(-1, 4)

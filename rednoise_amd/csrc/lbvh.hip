@@ -1,0 +1,644 @@
+// GPU LBVH build: centroid bounds -> 30-bit Morton codes -> stable LSD radix
+// sort (wave64 ballot ranking) -> Karras (HPG 2012) binary radix tree ->
+// bottom-up AABB refit with agent-scope atomics -> top-down collapse into the
+// compressed 8-wide BVH of rn_types.h (greedy largest-area opening, leaves of
+// <= 3 triangles, octant-ordered child slots, 8-bit conservative boxes).
+#include "rn_device.h"
+#include "rn_internal.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+
+namespace rn {
+
+namespace {
+
+constexpr int kLeafMax = 3;
+constexpr int kBlock = 256;
+
+// Total order on floats as unsigned ints (for atomic min/max of bounds).
+__device__ __forceinline__ uint32_t ford(float f) {
+	uint32_t u = __float_as_uint(f);
+	return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float fordinv(uint32_t u) {
+	return __uint_as_float((u & 0x80000000u) ? (u & 0x7FFFFFFFu) : ~u);
+}
+
+__device__ __forceinline__ float3 centroid(const float *v, int i) {
+#pragma clang fp contract(off)
+	const float *t = v + 9 * (size_t)i;
+	return f3((t[0] + t[3] + t[6]) / 3.0f, (t[1] + t[4] + t[7]) / 3.0f, (t[2] + t[5] + t[8]) / 3.0f);
+}
+
+__global__ void k_fill_u32(uint32_t *p, uint32_t v, size_t n) {
+	size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+	for (; i < n; i += (size_t)gridDim.x * blockDim.x)
+		p[i] = v;
+}
+
+__global__ void k_centroid_bounds(const float *v, int n, uint32_t *b) {
+	float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+	for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+		float3 c = centroid(v, i);
+		lo[0] = fminf(lo[0], c.x);
+		lo[1] = fminf(lo[1], c.y);
+		lo[2] = fminf(lo[2], c.z);
+		hi[0] = fmaxf(hi[0], c.x);
+		hi[1] = fmaxf(hi[1], c.y);
+		hi[2] = fmaxf(hi[2], c.z);
+	}
+#pragma unroll
+	for (int k = 0; k < 3; k++) {
+		for (int off = 32; off > 0; off >>= 1) {
+			lo[k] = fminf(lo[k], __shfl_xor(lo[k], off));
+			hi[k] = fmaxf(hi[k], __shfl_xor(hi[k], off));
+		}
+	}
+	if ((threadIdx.x & 63) == 0) {
+		for (int k = 0; k < 3; k++) {
+			atomicMin(&b[k], ford(lo[k]));
+			atomicMax(&b[3 + k], ford(hi[k]));
+		}
+	}
+}
+
+__device__ __forceinline__ uint32_t spread10(uint32_t x) { // bit i -> bit 3i
+	x &= 0x3FFu;
+	x = (x | (x << 16)) & 0x030000FFu;
+	x = (x | (x << 8)) & 0x0300F00Fu;
+	x = (x | (x << 4)) & 0x030C30C3u;
+	x = (x | (x << 2)) & 0x09249249u;
+	return x;
+}
+
+__global__ void k_morton(const float *v, int n, const uint32_t *b, uint32_t *keys, uint32_t *vals) {
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n)
+		return;
+	float3 c = centroid(v, i);
+	uint32_t q[3];
+#pragma unroll
+	for (int k = 0; k < 3; k++) {
+		float lo = fordinv(b[k]), hi = fordinv(b[3 + k]);
+		float ext = hi - lo;
+		float scale = ext > 0.0f ? 1024.0f / ext : 0.0f;
+		float f = (get3(c, k) - lo) * scale;
+		f = fminf(fmaxf(f, 0.0f), 1023.0f);
+		q[k] = (uint32_t)f;
+	}
+	keys[i] = spread10(q[0]) | (spread10(q[1]) << 1) | (spread10(q[2]) << 2);
+	vals[i] = (uint32_t)i;
+}
+
+// ------------------------------------------------------------ radix sort --
+
+constexpr int kRsItems = 8;
+constexpr int kRsTile = kBlock * kRsItems;
+
+__global__ void k_rs_hist(const uint32_t *keys, int n, int shift, uint32_t *hist, int nblocks) {
+	__shared__ uint32_t h[256];
+	h[threadIdx.x] = 0;
+	__syncthreads();
+	int base = blockIdx.x * kRsTile;
+#pragma unroll
+	for (int r = 0; r < kRsItems; r++) {
+		int p = base + r * kBlock + threadIdx.x;
+		if (p < n)
+			atomicAdd(&h[(keys[p] >> shift) & 0xFFu], 1u);
+	}
+	__syncthreads();
+	hist[(size_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+}
+
+// Stable scatter. Tile position p = base + r*256 + tid; within a round the
+// lanes of a wave are ranked by an 8-ballot digit match, waves are ordered by
+// per-digit prefix sums in LDS, rounds by a running per-digit count.
+__global__ void k_rs_scatter(const uint32_t *keys, const uint32_t *vals, int n, int shift, const uint32_t *offs,
+                             int nblocks, uint32_t *keys_out, uint32_t *vals_out) {
+	__shared__ uint32_t run[256];
+	__shared__ uint32_t wc[4][256];
+	__shared__ uint32_t wb[4][256];
+	const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+	run[tid] = 0;
+	for (int w = 0; w < 4; w++)
+		wc[w][tid] = 0;
+	__syncthreads();
+	const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+	const int base = blockIdx.x * kRsTile;
+	const uint32_t gbase = offs[(size_t)tid * nblocks + blockIdx.x];
+	__shared__ uint32_t gb[256];
+	gb[tid] = gbase;
+	for (int r = 0; r < kRsItems; r++) {
+		int p = base + r * kBlock + tid;
+		bool valid = p < n;
+		uint32_t key = valid ? keys[p] : 0u, val = valid ? vals[p] : 0u;
+		uint32_t digit = (key >> shift) & 0xFFu;
+		uint64_t m = __ballot(valid);
+#pragma unroll
+		for (int b = 0; b < 8; b++) {
+			uint64_t bb = __ballot((digit >> b) & 1u);
+			m &= ((digit >> b) & 1u) ? bb : ~bb;
+		}
+		uint32_t rank = (uint32_t)__popcll(m & lt);
+		uint32_t cnt = (uint32_t)__popcll(m);
+		if (valid && rank == cnt - 1)
+			wc[wave][digit] = cnt;
+		__syncthreads();
+		uint32_t acc = run[tid];
+#pragma unroll
+		for (int w = 0; w < 4; w++) {
+			uint32_t c = wc[w][tid];
+			wb[w][tid] = acc;
+			acc += c;
+			wc[w][tid] = 0;
+		}
+		run[tid] = acc;
+		__syncthreads();
+		if (valid) {
+			uint32_t dst = gb[digit] + wb[wave][digit] + rank;
+			keys_out[dst] = key;
+			vals_out[dst] = val;
+		}
+	}
+}
+
+// Exclusive scan of uint32 (1024 elements per block, recursive over blocks).
+constexpr int kScanItems = 4;
+constexpr int kScanTile = kBlock * kScanItems;
+
+__global__ void k_scan_tile(const uint32_t *in, uint32_t *out, size_t n, uint32_t *sums) {
+	__shared__ uint32_t s[kBlock];
+	size_t base = (size_t)blockIdx.x * kScanTile + (size_t)threadIdx.x * kScanItems;
+	uint32_t v[kScanItems], tot = 0;
+#pragma unroll
+	for (int k = 0; k < kScanItems; k++) {
+		v[k] = (base + k < n) ? in[base + k] : 0u;
+		tot += v[k];
+	}
+	s[threadIdx.x] = tot;
+	__syncthreads();
+	for (int off = 1; off < kBlock; off <<= 1) {
+		uint32_t x = threadIdx.x >= (unsigned)off ? s[threadIdx.x - off] : 0u;
+		__syncthreads();
+		s[threadIdx.x] += x;
+		__syncthreads();
+	}
+	uint32_t excl = s[threadIdx.x] - tot;
+#pragma unroll
+	for (int k = 0; k < kScanItems; k++) {
+		if (base + k < n)
+			out[base + k] = excl;
+		excl += v[k];
+	}
+	if (threadIdx.x == kBlock - 1 && sums)
+		sums[blockIdx.x] = s[kBlock - 1];
+}
+
+__global__ void k_scan_add(uint32_t *out, size_t n, const uint32_t *sums) {
+	size_t i = (size_t)blockIdx.x * kScanTile + threadIdx.x;
+	uint32_t add = sums[blockIdx.x];
+	for (int k = 0; k < kScanItems; k++, i += kBlock)
+		if (i < n)
+			out[i] += add;
+}
+
+void exclusive_scan(const uint32_t *in, uint32_t *out, size_t n, uint32_t *tmp, hipStream_t s) {
+	size_t blocks = (n + kScanTile - 1) / kScanTile;
+	if (blocks <= 1) {
+		k_scan_tile<<<1, kBlock, 0, s>>>(in, out, n, nullptr);
+		return;
+	}
+	uint32_t *sums = tmp, *sums_scan = tmp + blocks, *rest = tmp + 2 * blocks;
+	k_scan_tile<<<(unsigned)blocks, kBlock, 0, s>>>(in, out, n, sums);
+	exclusive_scan(sums, sums_scan, blocks, rest, s);
+	k_scan_add<<<(unsigned)blocks, kBlock, 0, s>>>(out, n, sums_scan);
+}
+
+size_t scan_tmp_words(size_t n) {
+	size_t total = 0;
+	while (true) {
+		size_t blocks = (n + kScanTile - 1) / kScanTile;
+		if (blocks <= 1)
+			break;
+		total += 2 * blocks;
+		n = blocks;
+	}
+	return total + 16;
+}
+
+// ---------------------------------------------------------------- Karras --
+
+__device__ __forceinline__ int delta(const uint32_t *k, int n, int i, int j) {
+	if (j < 0 || j >= n)
+		return -1;
+	uint32_t a = k[i], b = k[j];
+	if (a == b)
+		return 32 + __clz((uint32_t)(i ^ j));
+	return __clz(a ^ b);
+}
+
+// Internal node i of n-1; parent[] holds internal nodes at [0, n-1) and leaf
+// p at n-1+p; children >= 0 are internal, < 0 are leaves ~p.
+__global__ void k_karras(const uint32_t *k, int n, int *left, int *right, int *parent, int2 *range) {
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n - 1)
+		return;
+	int d = (delta(k, n, i, i + 1) - delta(k, n, i, i - 1)) >= 0 ? 1 : -1;
+	int dmin = delta(k, n, i, i - d);
+	int lmax = 2;
+	while (delta(k, n, i, i + lmax * d) > dmin)
+		lmax <<= 1;
+	int l = 0;
+	for (int t = lmax >> 1; t >= 1; t >>= 1)
+		if (delta(k, n, i, i + (l + t) * d) > dmin)
+			l += t;
+	int j = i + l * d;
+	int dnode = delta(k, n, i, j);
+	int s = 0, t = l;
+	do {
+		t = (t + 1) >> 1;
+		if (delta(k, n, i, i + (s + t) * d) > dnode)
+			s += t;
+	} while (t > 1);
+	int g = i + s * d + (d < 0 ? -1 : 0);
+	int lo = min(i, j), hi = max(i, j);
+	int lc = (lo == g) ? ~g : g;
+	int rc = (hi == g + 1) ? ~(g + 1) : g + 1;
+	left[i] = lc;
+	right[i] = rc;
+	parent[lc >= 0 ? lc : n - 1 + ~lc] = i;
+	parent[rc >= 0 ? rc : n - 1 + ~rc] = i;
+	range[i] = make_int2(lo, hi);
+	if (i == 0)
+		parent[0] = -1;
+}
+
+__global__ void k_refit(const float *v, const uint32_t *order, int n, const int *parent, const int *left,
+                        const int *right, float4 *box, uint32_t *visit) {
+	int p = blockIdx.x * blockDim.x + threadIdx.x;
+	if (p >= n)
+		return;
+	const float *t = v + 9 * (size_t)order[p];
+	float4 lo = make_float4(fminf(fminf(t[0], t[3]), t[6]), fminf(fminf(t[1], t[4]), t[7]),
+	                        fminf(fminf(t[2], t[5]), t[8]), 0.0f);
+	float4 hi = make_float4(fmaxf(fmaxf(t[0], t[3]), t[6]), fmaxf(fmaxf(t[1], t[4]), t[7]),
+	                        fmaxf(fmaxf(t[2], t[5]), t[8]), 0.0f);
+	size_t li = (size_t)(n - 1 + p);
+	box[2 * li] = lo;
+	box[2 * li + 1] = hi;
+	if (n < 2)
+		return;
+	int node = parent[li];
+	while (node >= 0) {
+		uint32_t old = __hip_atomic_fetch_add(&visit[node], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+		if (old == 0)
+			return;
+		int l = left[node], r = right[node];
+		size_t a = l >= 0 ? (size_t)l : (size_t)(n - 1 + ~l);
+		size_t b = r >= 0 ? (size_t)r : (size_t)(n - 1 + ~r);
+		float4 al = box[2 * a], ah = box[2 * a + 1], bl = box[2 * b], bh = box[2 * b + 1];
+		lo = make_float4(fminf(al.x, bl.x), fminf(al.y, bl.y), fminf(al.z, bl.z), 0.0f);
+		hi = make_float4(fmaxf(ah.x, bh.x), fmaxf(ah.y, bh.y), fmaxf(ah.z, bh.z), 0.0f);
+		box[2 * (size_t)node] = lo;
+		box[2 * (size_t)node + 1] = hi;
+		node = parent[node];
+	}
+}
+
+// -------------------------------------------------------------- collapse --
+
+struct Entry {
+	int id; // >= 0 binary internal, < 0 leaf ~pos
+	float3 lo, hi;
+	int first, last;
+};
+
+__device__ __forceinline__ Entry make_entry(int c, int n, const float4 *box, const int2 *range) {
+	Entry e;
+	e.id = c;
+	size_t bi = c >= 0 ? (size_t)c : (size_t)(n - 1 + ~c);
+	float4 lo = box[2 * bi], hi = box[2 * bi + 1];
+	e.lo = f3(lo.x, lo.y, lo.z);
+	e.hi = f3(hi.x, hi.y, hi.z);
+	if (c >= 0) {
+		int2 r = range[c];
+		e.first = r.x;
+		e.last = r.y;
+	} else {
+		e.first = e.last = ~c;
+	}
+	return e;
+}
+
+__device__ __forceinline__ float half_area(const Entry &e) {
+	float3 d = e.hi - e.lo;
+	return d.x * d.y + d.y * d.z + d.z * d.x;
+}
+
+// Smallest power of two s with 255 * s >= ext, as an exponent byte (>= 1).
+__device__ __forceinline__ uint32_t scale_exp(float ext) {
+	float v = ext / 255.0f;
+	uint32_t bits = __float_as_uint(v);
+	uint32_t e = (bits >> 23) & 0xFFu;
+	if (bits & 0x7FFFFFu)
+		e += 1;
+	return e < 1 ? 1u : e;
+}
+
+__global__ void k_collapse(int k0, int k1, int width, int *task, const float4 *box, const int *left, const int *right,
+                           const int2 *range, const uint32_t *order, const float *verts, int n, Bvh8Node *nodes,
+                           float4 *tris, uint32_t *counters, int node_cap) {
+	int k = k0 + blockIdx.x * blockDim.x + threadIdx.x;
+	if (k >= k1)
+		return;
+	int b = task[k];
+	Entry e[8];
+	int m;
+	if (b < 0) {
+		e[0] = make_entry(b, n, box, range);
+		m = 1;
+	} else {
+		e[0] = make_entry(left[b], n, box, range);
+		e[1] = make_entry(right[b], n, box, range);
+		m = 2;
+		while (m < width) {
+			int best = -1;
+			float ba = -1.0f;
+			for (int j = 0; j < m; j++) {
+				if (e[j].id >= 0 && e[j].last - e[j].first + 1 > kLeafMax) {
+					float a = half_area(e[j]);
+					if (a > ba) {
+						ba = a;
+						best = j;
+					}
+				}
+			}
+			if (best < 0)
+				break;
+			int id = e[best].id;
+			e[best] = make_entry(left[id], n, box, range);
+			e[m++] = make_entry(right[id], n, box, range);
+		}
+	}
+	float3 lo = e[0].lo, hi = e[0].hi;
+	for (int j = 1; j < m; j++) {
+		lo = f3(fminf(lo.x, e[j].lo.x), fminf(lo.y, e[j].lo.y), fminf(lo.z, e[j].lo.z));
+		hi = f3(fmaxf(hi.x, e[j].hi.x), fmaxf(hi.y, e[j].hi.y), fmaxf(hi.z, e[j].hi.z));
+	}
+	// Octant slot assignment: greedily give each (entry, slot) pair with the
+	// largest projection of the entry centre onto the slot diagonal.
+	float3 c = 0.5f * (lo + hi);
+	int slot_entry[8];
+	for (int s = 0; s < 8; s++)
+		slot_entry[s] = -1;
+	uint32_t used = 0;
+	for (int it = 0; it < m; it++) {
+		float bs = -INFINITY;
+		int bj = -1, bsl = -1;
+		for (int j = 0; j < m; j++) {
+			if (used & (1u << j))
+				continue;
+			float3 dc = 0.5f * (e[j].lo + e[j].hi) - c;
+			for (int s = 0; s < 8; s++) {
+				if (slot_entry[s] >= 0)
+					continue;
+				float sc = ((s & 1) ? dc.x : -dc.x) + ((s & 2) ? dc.y : -dc.y) + ((s & 4) ? dc.z : -dc.z);
+				if (sc > bs) {
+					bs = sc;
+					bj = j;
+					bsl = s;
+				}
+			}
+		}
+		used |= 1u << bj;
+		slot_entry[bsl] = bj;
+	}
+	int n_int = 0, n_tri = 0;
+	for (int j = 0; j < m; j++) {
+		int cnt = e[j].last - e[j].first + 1;
+		if (cnt > kLeafMax)
+			n_int++;
+		else
+			n_tri += cnt;
+	}
+	uint32_t child_base = n_int ? atomicAdd(&counters[0], (uint32_t)n_int) : 0u;
+	uint32_t tri_base = n_tri ? atomicAdd(&counters[1], (uint32_t)n_tri) : 0u;
+	if (child_base + (uint32_t)n_int > (uint32_t)node_cap) {
+		atomicOr(&counters[2], 1u); // capacity overflow: the host rebuilds with more room
+		return;
+	}
+	if (tri_base + (uint32_t)n_tri > (uint32_t)n) {
+		atomicOr(&counters[2], 2u); // more leaf triangles than input: malformed hierarchy
+		return;
+	}
+	Bvh8Node nd;
+	uint32_t ex = scale_exp(hi.x - lo.x), ey = scale_exp(hi.y - lo.y), ez = scale_exp(hi.z - lo.z);
+	double sx = ldexp(1.0, (int)ex - 127), sy = ldexp(1.0, (int)ey - 127), sz = ldexp(1.0, (int)ez - 127);
+	nd.p[0] = lo.x;
+	nd.p[1] = lo.y;
+	nd.p[2] = lo.z;
+	nd.e[0] = (uint8_t)ex;
+	nd.e[1] = (uint8_t)ey;
+	nd.e[2] = (uint8_t)ez;
+	nd.child_base = child_base;
+	nd.tri_base = tri_base;
+	uint32_t imask = 0, ic = 0, off = 0;
+	for (int s = 0; s < 8; s++) {
+		int j = slot_entry[s];
+		nd.meta[s] = 0;
+		if (j < 0) {
+			for (int a = 0; a < 3; a++) {
+				nd.qlo[a][s] = 255;
+				nd.qhi[a][s] = 0;
+			}
+			continue;
+		}
+		const Entry &q = e[j];
+		double qlo[3] = {floor(((double)q.lo.x - (double)lo.x) / sx), floor(((double)q.lo.y - (double)lo.y) / sy),
+		                 floor(((double)q.lo.z - (double)lo.z) / sz)};
+		double qhi[3] = {ceil(((double)q.hi.x - (double)lo.x) / sx), ceil(((double)q.hi.y - (double)lo.y) / sy),
+		                 ceil(((double)q.hi.z - (double)lo.z) / sz)};
+		for (int a = 0; a < 3; a++) {
+			nd.qlo[a][s] = (uint8_t)fmin(fmax(qlo[a], 0.0), 255.0);
+			nd.qhi[a][s] = (uint8_t)fmin(fmax(qhi[a], 0.0), 255.0);
+		}
+		int cnt = q.last - q.first + 1;
+		if (cnt > kLeafMax) {
+			imask |= 1u << s;
+			task[child_base + ic] = q.id;
+			ic++;
+		} else {
+			nd.meta[s] = (uint8_t)((cnt << 5) | off);
+			for (int r = 0; r < cnt; r++) {
+				uint32_t prim = order[q.first + r];
+				const float *t = verts + 9 * (size_t)prim;
+				size_t dst = 3 * (size_t)(tri_base + off + r);
+				tris[dst] = make_float4(t[0], t[1], t[2], __int_as_float((int)prim));
+				tris[dst + 1] = make_float4(t[3], t[4], t[5], 0.0f);
+				tris[dst + 2] = make_float4(t[6], t[7], t[8], 0.0f);
+			}
+			off += cnt;
+		}
+	}
+	nd.imask = (uint8_t)imask;
+	const uint4 *src = reinterpret_cast<const uint4 *>(&nd);
+	uint4 *dst = reinterpret_cast<uint4 *>(nodes + k);
+	for (int q = 0; q < 5; q++)
+		dst[q] = src[q];
+}
+
+__global__ void k_empty_root(Bvh8Node *nodes) {
+	Bvh8Node nd;
+	for (int a = 0; a < 3; a++) {
+		nd.p[a] = 0.0f;
+		nd.e[a] = 127;
+		for (int s = 0; s < 8; s++) {
+			nd.qlo[a][s] = 255;
+			nd.qhi[a][s] = 0;
+		}
+	}
+	for (int s = 0; s < 8; s++)
+		nd.meta[s] = 0;
+	nd.imask = 0;
+	nd.child_base = 0;
+	nd.tri_base = 0;
+	*nodes = nd;
+}
+
+__global__ void k_set_root_task(int *task, int n, uint32_t *counters) {
+	task[0] = n >= 2 ? 0 : ~0;
+	counters[0] = 1;
+	counters[1] = 0;
+	counters[2] = 0;
+}
+
+unsigned grid_for(size_t n, int block = kBlock) { return (unsigned)std::max<size_t>(1, (n + block - 1) / block); }
+
+void fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s) {
+	k_fill_u32<<<(unsigned)std::min<size_t>(4096, grid_for(n)), kBlock, 0, s>>>(p, v, n);
+}
+
+// Codes + sorted order + (n >= 2) binary tree and refitted boxes.
+void lbvh_front(Bvh &b, const float *d_verts, int n, hipStream_t s) {
+	uint32_t *bounds = static_cast<uint32_t *>(b.bounds.reserve(8 * sizeof(uint32_t)));
+	uint32_t *keys = static_cast<uint32_t *>(b.keys.reserve((size_t)n * 4));
+	uint32_t *vals = static_cast<uint32_t *>(b.vals.reserve((size_t)n * 4));
+	uint32_t *keys2 = static_cast<uint32_t *>(b.keys2.reserve((size_t)n * 4));
+	uint32_t *vals2 = static_cast<uint32_t *>(b.vals2.reserve((size_t)n * 4));
+	fill_u32(bounds, 0xFFFFFFFFu, 3, s);
+	fill_u32(bounds + 3, 0u, 3, s);
+	k_centroid_bounds<<<std::min(1024u, grid_for(n)), kBlock, 0, s>>>(d_verts, n, bounds);
+	k_morton<<<grid_for(n), kBlock, 0, s>>>(d_verts, n, bounds, keys, vals);
+	radix_sort_pairs(keys, vals, keys2, vals2, n, 30, b.hist, b.scan_tmp, s);
+	size_t nn = (size_t)(n > 1 ? 2 * n - 1 : 1);
+	int *left = static_cast<int *>(b.left.reserve((size_t)std::max(1, n - 1) * 4));
+	int *right = static_cast<int *>(b.right.reserve((size_t)std::max(1, n - 1) * 4));
+	int *parent = static_cast<int *>(b.parent.reserve(nn * 4));
+	int2 *range = static_cast<int2 *>(b.range.reserve((size_t)std::max(1, n - 1) * 8));
+	float4 *box = static_cast<float4 *>(b.boxes.reserve(nn * 32));
+	uint32_t *visit = static_cast<uint32_t *>(b.visit.reserve((size_t)std::max(1, n - 1) * 4));
+	if (n >= 2) {
+		k_karras<<<grid_for(n - 1), kBlock, 0, s>>>(keys, n, left, right, parent, range);
+		fill_u32(visit, 0u, (size_t)(n - 1), s);
+	}
+	k_refit<<<grid_for(n), kBlock, 0, s>>>(d_verts, vals, n, parent, left, right, box, visit);
+}
+
+} // namespace
+
+void radix_sort_pairs(uint32_t *d_keys, uint32_t *d_vals, uint32_t *d_keys_tmp, uint32_t *d_vals_tmp, int n,
+                      int bits, DevBuf &hist, DevBuf &scan_tmp, hipStream_t s) {
+	if (n <= 1)
+		return;
+	int nblocks = (n + kRsTile - 1) / kRsTile;
+	size_t hn = (size_t)256 * nblocks;
+	uint32_t *h = static_cast<uint32_t *>(hist.reserve(2 * hn * 4));
+	uint32_t *offs = h + hn;
+	uint32_t *tmp = static_cast<uint32_t *>(scan_tmp.reserve(scan_tmp_words(hn) * 4));
+	uint32_t *ki = d_keys, *vi = d_vals, *ko = d_keys_tmp, *vo = d_vals_tmp;
+	int passes = (bits + 7) / 8;
+	for (int p = 0; p < passes; p++) {
+		k_rs_hist<<<nblocks, kBlock, 0, s>>>(ki, n, 8 * p, h, nblocks);
+		exclusive_scan(h, offs, hn, tmp, s);
+		k_rs_scatter<<<nblocks, kBlock, 0, s>>>(ki, vi, n, 8 * p, offs, nblocks, ko, vo);
+		std::swap(ki, ko);
+		std::swap(vi, vo);
+	}
+	if (ki != d_keys) {
+		RN_HIP(hipMemcpyAsync(d_keys, ki, (size_t)n * 4, hipMemcpyDeviceToDevice, s));
+		RN_HIP(hipMemcpyAsync(d_vals, vi, (size_t)n * 4, hipMemcpyDeviceToDevice, s));
+	}
+}
+
+void build_bvh(Bvh &b, const float *d_verts, int n, int width, hipStream_t s) {
+	auto t0 = std::chrono::steady_clock::now();
+	if (width != 4 && width != 8)
+		throw Error("bvh width must be 4 or 8");
+	b.width = width;
+	b.n_tris = n;
+	uint32_t *counters = static_cast<uint32_t *>(b.counters.reserve(4 * sizeof(uint32_t)));
+	if (n == 0) {
+		Bvh8Node *nodes = static_cast<Bvh8Node *>(b.nodes.reserve(sizeof(Bvh8Node)));
+		b.tris.reserve(48);
+		k_empty_root<<<1, 1, 0, s>>>(nodes);
+		RN_HIP(hipStreamSynchronize(s));
+		b.n_nodes = 1;
+		b.depth = 1;
+		b.build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+		return;
+	}
+	lbvh_front(b, d_verts, n, s);
+	float4 *tris = static_cast<float4 *>(b.tris.reserve((size_t)n * 48));
+	int cap = n / 2 + 64;
+	for (int attempt = 0; attempt < 3; attempt++) {
+		Bvh8Node *nodes = static_cast<Bvh8Node *>(b.nodes.reserve((size_t)cap * sizeof(Bvh8Node)));
+		int *task = static_cast<int *>(b.task.reserve((size_t)cap * 4));
+		k_set_root_task<<<1, 1, 0, s>>>(task, n, counters);
+		uint32_t lo = 0, hi = 1, h[4];
+		int depth = 0;
+		bool overflow = false;
+		while (lo < hi) {
+			k_collapse<<<grid_for(hi - lo, 64), 64, 0, s>>>(
+			    (int)lo, (int)hi, width, task, b.boxes.as<float4>(), b.left.as<int>(), b.right.as<int>(),
+			    b.range.as<int2>(), b.vals.as<uint32_t>(), d_verts, n, nodes, tris, counters, cap);
+			RN_HIP(hipMemcpyAsync(h, counters, 3 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+			RN_HIP(hipStreamSynchronize(s));
+			depth++;
+			if (h[2] & 2u)
+				throw Error("BVH8 collapse produced more leaf triangles than the input has");
+			if (h[2]) {
+				overflow = true;
+				break;
+			}
+			lo = hi;
+			hi = h[0];
+		}
+		if (!overflow) {
+			if ((int)h[1] != n)
+				throw Error("BVH8 collapse lost triangles");
+			b.n_nodes = (int)hi;
+			b.depth = depth;
+			b.build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+			return;
+		}
+		cap = 2 * cap + n;
+	}
+	throw Error("BVH8 collapse exceeded its node capacity");
+}
+
+void lbvh_stages(const float *d_verts, int n, uint32_t *h_codes, int *h_order, int *h_left, int *h_right,
+                 hipStream_t s) {
+	Bvh b;
+	lbvh_front(b, d_verts, n, s);
+	RN_HIP(hipMemcpyAsync(h_codes, b.keys.ptr, (size_t)n * 4, hipMemcpyDeviceToHost, s));
+	RN_HIP(hipMemcpyAsync(h_order, b.vals.ptr, (size_t)n * 4, hipMemcpyDeviceToHost, s));
+	if (n >= 2) {
+		RN_HIP(hipMemcpyAsync(h_left, b.left.ptr, (size_t)(n - 1) * 4, hipMemcpyDeviceToHost, s));
+		RN_HIP(hipMemcpyAsync(h_right, b.right.ptr, (size_t)(n - 1) * 4, hipMemcpyDeviceToHost, s));
+	}
+	RN_HIP(hipStreamSynchronize(s));
+}
+
+} // namespace rn

@@ -1,0 +1,339 @@
+// Device-side building blocks shared by the kernels: vector helpers, Philox,
+// the BVH8 node layout, the two ray/triangle tests and wide-BVH traversal.
+// Everything here is gfx950 (CDNA4, wave64) code; there is no other target.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "rn_types.h"
+
+#define RN_DEV __device__ __forceinline__
+
+// ------------------------------------------------------------------ math --
+
+RN_DEV float3 f3(float x, float y, float z) { return make_float3(x, y, z); }
+RN_DEV float3 operator+(float3 a, float3 b) { return f3(a.x + b.x, a.y + b.y, a.z + b.z); }
+RN_DEV float3 operator-(float3 a, float3 b) { return f3(a.x - b.x, a.y - b.y, a.z - b.z); }
+RN_DEV float3 operator*(float3 a, float3 b) { return f3(a.x * b.x, a.y * b.y, a.z * b.z); }
+RN_DEV float3 operator*(float3 a, float s) { return f3(a.x * s, a.y * s, a.z * s); }
+RN_DEV float3 operator*(float s, float3 a) { return f3(a.x * s, a.y * s, a.z * s); }
+RN_DEV float dot3(float3 a, float3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+RN_DEV float3 cross3(float3 a, float3 b) {
+	return f3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+RN_DEV float3 normalize3(float3 a) { return a * (1.0f / sqrtf(dot3(a, a))); }
+RN_DEV float maxc3(float3 a) { return fmaxf(fmaxf(a.x, a.y), a.z); }
+RN_DEV float get3(float3 a, int k) { return k == 0 ? a.x : (k == 1 ? a.y : a.z); }
+RN_DEV float3 xyz(float4 a) { return f3(a.x, a.y, a.z); }
+
+// ---------------------------------------------------------------- Philox --
+// Philox4x32-10 (Salmon et al., SC'11): ten rounds of two 32x32->64-bit
+// multiplies; mul_hi/mul_lo map to v_mul_hi_u32 / v_mul_lo_u32.
+
+struct u32x4 {
+	uint32_t x, y, z, w;
+};
+
+RN_DEV u32x4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1) {
+#pragma unroll
+	for (int r = 0; r < 10; r++) {
+		if (r > 0) {
+			k0 += 0x9E3779B9u;
+			k1 += 0xBB67AE85u;
+		}
+		uint32_t hi0 = __umulhi(0xD2511F53u, c0), lo0 = 0xD2511F53u * c0;
+		uint32_t hi1 = __umulhi(0xCD9E8D57u, c2), lo1 = 0xCD9E8D57u * c2;
+		uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+		c0 = n0;
+		c1 = lo1;
+		c2 = n2;
+		c3 = lo0;
+	}
+	return u32x4{c0, c1, c2, c3};
+}
+
+// 24-bit uniform in [0,1): exactly representable, identical on host and device.
+RN_DEV float u01(uint32_t x) { return (float)(x >> 8) * (1.0f / 16777216.0f); }
+
+// ------------------------------------------------------------------ BVH8 --
+
+RN_DEV void load_node(const Bvh8Node *__restrict__ nodes, uint32_t i, uint32_t w[20]) {
+	const uint4 *p = reinterpret_cast<const uint4 *>(nodes + i);
+#pragma unroll
+	for (int k = 0; k < 5; k++) {
+		uint4 q = p[k];
+		w[4 * k] = q.x;
+		w[4 * k + 1] = q.y;
+		w[4 * k + 2] = q.z;
+		w[4 * k + 3] = q.w;
+	}
+}
+
+// Ray state shared by the traversal loop.
+struct TRay {
+	float3 o, d;
+	float3 inv;  // 1/d with zero components replaced by +-1e-30
+	uint32_t oct; // bit k set when d[k] < 0
+};
+
+RN_DEV TRay make_tray(float3 o, float3 d) {
+	TRay r;
+	r.o = o;
+	r.d = d;
+	const float tiny = 1e-30f;
+	float dx = fabsf(d.x) > tiny ? d.x : copysignf(tiny, d.x);
+	float dy = fabsf(d.y) > tiny ? d.y : copysignf(tiny, d.y);
+	float dz = fabsf(d.z) > tiny ? d.z : copysignf(tiny, d.z);
+	r.inv = f3(1.0f / dx, 1.0f / dy, 1.0f / dz);
+	r.oct = (d.x < 0.0f ? 1u : 0u) | (d.y < 0.0f ? 2u : 0u) | (d.z < 0.0f ? 4u : 0u);
+	return r;
+}
+
+// Intersect the (up to) 8 child boxes of a node. Returns the internal-child hit
+// bits in traversal order (bit b <-> slot b ^ oct, so bit 0 is the slot built
+// for the ray's own octant, i.e. the nearest) and the leaf triangle bits.
+RN_DEV void intersect_children(const uint32_t w[20], const TRay &r, float tmax, uint32_t &ihits, uint32_t &thits) {
+	float px = __uint_as_float(w[0]), py = __uint_as_float(w[1]), pz = __uint_as_float(w[2]);
+	uint32_t e = w[3];
+	float sx = __uint_as_float((e & 0xFFu) << 23);
+	float sy = __uint_as_float(((e >> 8) & 0xFFu) << 23);
+	float sz = __uint_as_float(((e >> 16) & 0xFFu) << 23);
+	uint32_t imask = e >> 24;
+	float ax = (px - r.o.x) * r.inv.x, ay = (py - r.o.y) * r.inv.y, az = (pz - r.o.z) * r.inv.z;
+	float bx = sx * r.inv.x, by = sy * r.inv.y, bz = sz * r.inv.z;
+	// words: 4 child_base, 5 tri_base, 6-7 meta, 8-9 qlo.x, 10-11 qlo.y,
+	// 12-13 qlo.z, 14-15 qhi.x, 16-17 qhi.y, 18-19 qhi.z
+	uint32_t nx0 = r.inv.x >= 0 ? w[8] : w[14], nx1 = r.inv.x >= 0 ? w[9] : w[15];
+	uint32_t fx0 = r.inv.x >= 0 ? w[14] : w[8], fx1 = r.inv.x >= 0 ? w[15] : w[9];
+	uint32_t ny0 = r.inv.y >= 0 ? w[10] : w[16], ny1 = r.inv.y >= 0 ? w[11] : w[17];
+	uint32_t fy0 = r.inv.y >= 0 ? w[16] : w[10], fy1 = r.inv.y >= 0 ? w[17] : w[11];
+	uint32_t nz0 = r.inv.z >= 0 ? w[12] : w[18], nz1 = r.inv.z >= 0 ? w[13] : w[19];
+	uint32_t fz0 = r.inv.z >= 0 ? w[18] : w[12], fz1 = r.inv.z >= 0 ? w[19] : w[13];
+	const float tfar_lim = tmax * 1.0000005f;
+	ihits = 0;
+	thits = 0;
+#pragma unroll
+	for (int s = 0; s < 8; s++) {
+		const int sh = (s & 3) * 8;
+		uint32_t nxw = s < 4 ? nx0 : nx1, fxw = s < 4 ? fx0 : fx1;
+		uint32_t nyw = s < 4 ? ny0 : ny1, fyw = s < 4 ? fy0 : fy1;
+		uint32_t nzw = s < 4 ? nz0 : nz1, fzw = s < 4 ? fz0 : fz1;
+		float tnx = fmaf((float)((nxw >> sh) & 0xFFu), bx, ax);
+		float tny = fmaf((float)((nyw >> sh) & 0xFFu), by, ay);
+		float tnz = fmaf((float)((nzw >> sh) & 0xFFu), bz, az);
+		float tfx = fmaf((float)((fxw >> sh) & 0xFFu), bx, ax);
+		float tfy = fmaf((float)((fyw >> sh) & 0xFFu), by, ay);
+		float tfz = fmaf((float)((fzw >> sh) & 0xFFu), bz, az);
+		float tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, 0.0f));
+		float tf = fminf(fminf(tfx, tfy), fminf(tfz, tfar_lim));
+		bool hit = tn <= tf * 1.0000005f;
+		uint32_t meta = (s < 4 ? w[6] : w[7]) >> sh & 0xFFu;
+		bool internal = (imask >> s) & 1u;
+		if (hit && internal)
+			ihits |= 1u << (s ^ r.oct);
+		if (hit && meta != 0u) {
+			uint32_t cnt = meta >> 5, off = meta & 31u;
+			thits |= ((1u << cnt) - 1u) << off;
+		}
+	}
+}
+
+// -------------------------------------------------------- triangle tests --
+
+// Watertight test (Woop, Benthin, Wald, JCGT 2(1) 2013). Ray-constant shear
+// terms are computed once per ray.
+struct WRay {
+	int kx, ky, kz;
+	float sx, sy, sz;
+};
+
+RN_DEV WRay make_wray(float3 d) {
+	WRay w;
+	float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+	int kz = (ax > ay) ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);
+	int kx = kz == 2 ? 0 : kz + 1;
+	int ky = kx == 2 ? 0 : kx + 1;
+	if (get3(d, kz) < 0.0f) {
+		int t = kx;
+		kx = ky;
+		ky = t;
+	}
+	w.kx = kx;
+	w.ky = ky;
+	w.kz = kz;
+	float dz = get3(d, kz);
+	w.sx = get3(d, kx) / dz;
+	w.sy = get3(d, ky) / dz;
+	w.sz = 1.0f / dz;
+	return w;
+}
+
+// Returns true for a hit with 0 < t < tmax; u = weight of v1, v = weight of v2.
+RN_DEV bool tri_watertight(const WRay &w, float3 o, float3 v0, float3 v1, float3 v2, float tmax, float &t, float &u,
+                           float &v) {
+#pragma clang fp contract(off)
+	float3 A = v0 - o, B = v1 - o, C = v2 - o;
+	float Az0 = get3(A, w.kz), Bz0 = get3(B, w.kz), Cz0 = get3(C, w.kz);
+	float Ax = get3(A, w.kx) - w.sx * Az0, Ay = get3(A, w.ky) - w.sy * Az0;
+	float Bx = get3(B, w.kx) - w.sx * Bz0, By = get3(B, w.ky) - w.sy * Bz0;
+	float Cx = get3(C, w.kx) - w.sx * Cz0, Cy = get3(C, w.ky) - w.sy * Cz0;
+	float U = Cx * By - Cy * Bx, V = Ax * Cy - Ay * Cx, W = Bx * Ay - By * Ax;
+	if (U == 0.0f || V == 0.0f || W == 0.0f) {
+		U = (float)((double)Cx * (double)By - (double)Cy * (double)Bx);
+		V = (float)((double)Ax * (double)Cy - (double)Ay * (double)Cx);
+		W = (float)((double)Bx * (double)Ay - (double)By * (double)Ax);
+	}
+	if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f))
+		return false;
+	float det = U + V + W;
+	if (det == 0.0f)
+		return false;
+	float T = U * (w.sz * Az0) + V * (w.sz * Bz0) + W * (w.sz * Cz0);
+	if (det < 0.0f) {
+		T = -T;
+		det = -det;
+		V = -V;
+		W = -W;
+	}
+	if (!(T > 0.0f) || !(T < tmax * det))
+		return false;
+	float rcp = 1.0f / det;
+	t = T * rcp;
+	u = V * rcp;
+	v = W * rcp;
+	return true;
+}
+
+// The reference test (src/Geometry.cpp:11-25) in glm's FP32 operation order
+// (glm/detail/func_matrix.inl:253-262,322-344, type_mat3x3.inl:468-474), with
+// contraction off so (t,u,v) are bit-identical to the reference's x86 build.
+RN_DEV bool tri_compat(float3 o, float3 d, float3 v0, float3 v1, float3 v2, float &t, float &u, float &v) {
+#pragma clang fp contract(off)
+	float3 e0 = v1 - v0, e1 = v2 - v0, sp = o - v0;
+	float m00 = -d.x, m01 = -d.y, m02 = -d.z;
+	float m10 = e0.x, m11 = e0.y, m12 = e0.z;
+	float m20 = e1.x, m21 = e1.y, m22 = e1.z;
+	float c0 = m11 * m22 - m21 * m12, c1 = m01 * m22 - m21 * m02, c2 = m01 * m12 - m11 * m02;
+	float det = m00 * c0 - m10 * c1 + m20 * c2;
+	if (fabsf(det) < 1e-8f)
+		return false;
+	float ood = 1.0f / det;
+	float i00 = c0 * ood;
+	float i10 = -(m10 * m22 - m20 * m12) * ood;
+	float i20 = (m10 * m21 - m20 * m11) * ood;
+	float i01 = -c1 * ood;
+	float i11 = (m00 * m22 - m20 * m02) * ood;
+	float i21 = -(m00 * m21 - m20 * m01) * ood;
+	float i02 = c2 * ood;
+	float i12 = -(m00 * m12 - m10 * m02) * ood;
+	float i22 = (m00 * m11 - m10 * m01) * ood;
+	float rx = i00 * sp.x + i10 * sp.y + i20 * sp.z;
+	float ry = i01 * sp.x + i11 * sp.y + i21 * sp.z;
+	float rz = i02 * sp.x + i12 * sp.y + i22 * sp.z;
+	t = rx;
+	u = ry;
+	v = rz;
+	return (ry >= 0.0f) && (rz >= 0.0f) && (ry + rz <= 1.0f) && (rx > 1e-5f);
+}
+
+// ------------------------------------------------------------- traversal --
+
+// A node group on the stack: child_base | imask << 32 | pending hit bits << 40.
+RN_DEV uint64_t mk_group(uint32_t base, uint32_t imask, uint32_t hits) {
+	return (uint64_t)base | ((uint64_t)imask << 32) | ((uint64_t)hits << 40);
+}
+
+struct Hit {
+	int prim;      // original triangle id, -1 = miss (any-hit: 1 = occluded)
+	uint32_t leaf; // leaf-ordered triangle index of the hit
+	float t, u, v;
+};
+
+// Wide-BVH traversal, one ray per lane. The short stack lives in LDS: entry k
+// of lane `lane` is lds[k * ldsStride + lane]; entries beyond LDS_DEPTH spill
+// to a lane-private array of SPILL entries. The stack never holds more than
+// one entry per tree level below the root, so the host picks an instance with
+// LDS_DEPTH + SPILL >= depth - 1; a deeper push is dropped rather than
+// written out of bounds. QUERY 0 = closest hit, 1 = any hit (early exit).
+// TRI 0 = watertight, 1 = reference (compat) test. Triangles equal to
+// `ignore` (original id) are skipped.
+template <int QUERY, int TRI, int LDS_DEPTH, int SPILL>
+RN_DEV Hit traverse(const Bvh8Node *__restrict__ nodes, const float4 *__restrict__ tris, float3 o, float3 d,
+                    float tmax, int ignore, uint64_t *lds, int ldsStride, int lane) {
+	TRay r = make_tray(o, d);
+	WRay wr;
+	if (TRI == 0)
+		wr = make_wray(d);
+	Hit h;
+	h.prim = -1;
+	h.leaf = 0;
+	h.t = tmax;
+	h.u = 0.0f;
+	h.v = 0.0f;
+	uint64_t spill[SPILL > 0 ? SPILL : 1];
+	int sp = 0;
+	uint64_t grp = mk_group(0u, 1u, 1u << (0u ^ r.oct));
+	uint32_t tri_base = 0, thits = 0;
+	while (true) {
+		uint32_t hits = (uint32_t)(grp >> 40);
+		if (hits) {
+			uint32_t b = __builtin_ctz(hits);
+			hits &= hits - 1u;
+			uint32_t slot = b ^ r.oct;
+			uint32_t imask = (uint32_t)(grp >> 32) & 0xFFu;
+			uint32_t node = (uint32_t)grp + __builtin_popcount(imask & ((1u << slot) - 1u));
+			if (hits) {
+				uint64_t rest = (grp & 0xFFFFFFFFFFull) | ((uint64_t)hits << 40);
+				if (sp < LDS_DEPTH) {
+					lds[sp * ldsStride + lane] = rest;
+					sp++;
+				} else if (SPILL > 0 && sp < LDS_DEPTH + SPILL) {
+					spill[sp - LDS_DEPTH] = rest;
+					sp++;
+				}
+			}
+			uint32_t w[20];
+			load_node(nodes, node, w);
+			uint32_t ih;
+			intersect_children(w, r, h.t, ih, thits);
+			grp = mk_group(w[4], w[3] >> 24, ih);
+			tri_base = w[5];
+		}
+		while (thits) {
+			uint32_t k = __builtin_ctz(thits);
+			thits &= thits - 1u;
+			uint32_t ti = tri_base + k;
+			float4 a = tris[3 * ti], b4 = tris[3 * ti + 1], c4 = tris[3 * ti + 2];
+			int prim = __float_as_int(a.w);
+			if (prim == ignore)
+				continue;
+			float tt, uu, vv;
+			bool ok;
+			if (TRI == 0)
+				ok = tri_watertight(wr, o, xyz(a), xyz(b4), xyz(c4), h.t, tt, uu, vv);
+			else
+				ok = tri_compat(o, d, xyz(a), xyz(b4), xyz(c4), tt, uu, vv) && tt < h.t;
+			if (ok) {
+				if (QUERY == 1) {
+					h.prim = 1;
+					return h;
+				}
+				h.prim = prim;
+				h.leaf = ti;
+				h.t = tt;
+				h.u = uu;
+				h.v = vv;
+			}
+		}
+		if ((grp >> 40) == 0) {
+			if (sp == 0)
+				break;
+			sp--;
+			grp = (SPILL == 0 || sp < LDS_DEPTH) ? lds[sp * ldsStride + lane] : spill[sp - LDS_DEPTH];
+		}
+	}
+	if (QUERY == 1)
+		h.prim = 0;
+	return h;
+}

@@ -1,0 +1,60 @@
+// Host-side internals shared by the translation units of librednoise.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "rn_types.h"
+
+namespace rn {
+
+// Error raised anywhere below the C ABI; every extern "C" entry catches it and
+// turns it into a 0/NULL return plus rn_last_error().
+struct Error : std::runtime_error {
+	explicit Error(const std::string &m) : std::runtime_error(m) {}
+};
+
+void hip_check(hipError_t e, const char *what, const char *file, int line);
+#define RN_HIP(x) ::rn::hip_check((x), #x, __FILE__, __LINE__)
+
+// Grow-only device allocation owned by whoever holds it.
+struct DevBuf {
+	void *ptr = nullptr;
+	size_t bytes = 0;
+	DevBuf() = default;
+	DevBuf(const DevBuf &) = delete;
+	DevBuf &operator=(const DevBuf &) = delete;
+	~DevBuf() { release(); }
+	void release();
+	// Ensure capacity >= n bytes (contents are not preserved on growth).
+	void *reserve(size_t n);
+	template <class T> T *as() const { return static_cast<T *>(ptr); }
+};
+
+// BVH8 build products and scratch, owned by a scene and reused by rebuilds.
+struct Bvh {
+	DevBuf nodes, tris;
+	int n_nodes = 0, depth = 0, width = 8, n_tris = 0;
+	double build_ms = 0.0;
+	// scratch
+	DevBuf bounds, keys, vals, keys2, vals2, hist, scan_tmp, left, right, parent, range, boxes, visit, task, counters;
+};
+
+// Build the BVH of n triangles whose vertices (9n floats) are already on the
+// device. Stream-ordered; the call synchronises once per BVH level to learn
+// the level sizes.
+void build_bvh(Bvh &bvh, const float *d_verts, int n, int width, hipStream_t s);
+
+// Stages exposed for testing (device buffers in, host copies out).
+void lbvh_stages(const float *d_verts, int n, uint32_t *h_codes, int *h_order, int *h_left, int *h_right,
+                 hipStream_t s);
+void radix_sort_pairs(uint32_t *d_keys, uint32_t *d_vals, uint32_t *d_keys_tmp, uint32_t *d_vals_tmp, int n,
+                      int bits, DevBuf &hist, DevBuf &scan_tmp, hipStream_t s);
+
+} // namespace rn

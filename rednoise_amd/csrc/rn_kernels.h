@@ -1,0 +1,43 @@
+// Host launchers of the kernels (defined in the .hip files).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "rn_compat.h"
+#include "rn_types.h"
+
+namespace rn {
+
+// Traversal stack: kLdsDepth entries per lane in LDS, kSpillDepth more in a
+// lane-private array for BVHs deeper than kLdsDepth + 1 levels.
+constexpr int kLdsDepth = 12;
+constexpr int kSpillDepth = 52;
+constexpr int kMaxBvhDepth = kLdsDepth + kSpillDepth + 1;
+
+// Counters written by the integrators (device memory, 64-bit).
+struct DevCounters {
+	unsigned long long closest, shadow, paths, pad;
+};
+
+void launch_trace_rays(const DevScene &sc, int n, const float *o, const float *d, const float *tmax,
+                       const int *ignore, int query, int tri, int *prim, float *t, float *u, float *v, hipStream_t s);
+void launch_primary_compat(const DevScene &sc, const CompatDev &cd, int W, int H, float cam_z, int *prim, float *t,
+                           float *u, float *v, int *texel, hipStream_t s);
+void launch_philox(int n, const uint32_t *ctr, const uint32_t *key, uint32_t *out, hipStream_t s);
+
+// Plain per-pixel megakernel of the physical estimator: out = W*H*3 mean radiance.
+void launch_mega(const DevScene &sc, const DevCamera &cam, const DevParams &p, float *out, DevCounters *cnt,
+                 int block, hipStream_t s);
+
+// Path-replay adjoint. hot_slot[prim] in [0, kHotSlots) selects an LDS
+// accumulator for heavily hit triangles, -1 otherwise.
+constexpr int kHotSlots = 256;
+void launch_adjoint(const DevScene &sc, const DevCamera &cam, const DevParams &p, const float *dl_dimage,
+                    const int *hot_slot, const int *hot_prim, int n_hot, float *g_albedo, float *g_emission,
+                    hipStream_t s);
+
+// Reference estimator (rn_render RN_PATHTRACED): RGBA8 output.
+void launch_compat_render(const DevScene &sc, const CompatDev &cd, int W, int H, float cam_z, int spp, uint32_t key0,
+                          uint32_t key1, unsigned char *rgba, hipStream_t s);
+
+} // namespace rn

@@ -1,0 +1,127 @@
+"""Write golden vectors from the compiled reference (oracle/_ref/librednoise_ref.so).
+
+Run in the build container, where the reference sources exist:
+    make -C oracle ref && python tests/golden/make_golden.py
+Outputs (small .npz fixtures next to this script):
+  obj_<asset>.npz      loadOBJ triangles/normals/colours/materials/texcoords
+  primary_<asset>.npz  rn_render camera rays (160x120, cam_z 4; rays stored for
+                       every 37th pixel) and the
+                       BVH::intersect (prim, t, u, v) of each
+  rays_<asset>.npz     seeded random rays: BVH::intersect and
+                       getClosestIntersection results
+  modeR_cornell.npz    rn_render(RN_PATHTRACED) RGBA8 images of the Cornell
+                       asset (reference estimator, per-pixel mt19937)
+  noise.npz            fractalNoise(p, 4) at seeded points
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from oracle import oracle as O  # noqa: E402
+
+ASSETS = os.path.join(HERE, "assets")
+
+
+def load(ref, name, scale):
+    path = os.path.join(ASSETS, name).encode()
+    n = ref.ref_load_obj(path, scale, None, None, None, None, None, None)
+    v = np.empty(9 * n, np.float32)
+    nm = np.empty(3 * n, np.float32)
+    col = np.empty(3 * n, np.int32)
+    mat = np.empty(n, np.int32)
+    ht = np.empty(n, np.int32)
+    tc = np.empty(6 * n, np.float32)
+    ref.ref_load_obj(path, scale, v.ctypes.data, nm.ctypes.data, col.ctypes.data, mat.ctypes.data, ht.ctypes.data,
+                     tc.ctypes.data)
+    return dict(verts=v, normals=nm, colour=col, material=mat, has_texture=ht, texcoords=tc, scale=np.float32(scale))
+
+
+def bvh(ref, verts, o, d, ignore=None):
+    n = verts.size // 9
+    nr = o.size // 3
+    prim = np.empty(nr, np.int32)
+    t = np.empty(nr, np.float32)
+    u = np.empty(nr, np.float32)
+    v = np.empty(nr, np.float32)
+    ig = None if ignore is None else np.ascontiguousarray(ignore, np.int32).ctypes.data
+    ref.ref_bvh_intersect(verts, n, nr, o, d, ig, prim, t, u, v)
+    return prim, t, u, v
+
+
+def brute(ref, verts, o, d):
+    n = verts.size // 9
+    nr = o.size // 3
+    prim = np.empty(nr, np.int32)
+    t = np.empty(nr, np.float32)
+    u = np.empty(nr, np.float32)
+    v = np.empty(nr, np.float32)
+    ref.ref_brute_closest(verts, n, nr, o, d, prim, t, u, v)
+    return prim, t, u, v
+
+
+def main() -> None:
+    ref = O.ref()
+    if ref is None:
+        raise SystemExit("build the reference first: make -C oracle ref")
+    assets = {"cornell": ("cornell-box.obj", 0.35), "quad": ("textured-quad.obj", 0.35),
+              "sphere": ("sphere.obj", 1.0)}
+    for key, (name, scale) in assets.items():
+        ob = load(ref, name, scale)
+        np.savez_compressed(os.path.join(HERE, f"obj_{key}.npz"), **ob)
+        W, H, cz = 160, 120, 4.0
+        o = np.empty(3 * W * H, np.float32)
+        d = np.empty(3 * W * H, np.float32)
+        ref.ref_primary_rays(W, H, C.c_float(cz), None, o, d)
+        prim, t, u, v = bvh(ref, ob["verts"], o, d)
+        sub = np.arange(0, W * H, 37)
+        np.savez_compressed(os.path.join(HERE, f"primary_{key}.npz"), sub=sub, o_sub=o.reshape(-1, 3)[sub],
+                            d_sub=d.reshape(-1, 3)[sub], prim=prim, t=t, u=u, v=v, width=W, height=H,
+                            cam_z=np.float32(cz))
+        rng = np.random.default_rng(12345)
+        nr = 2048
+        vv = ob["verts"].reshape(-1, 3)
+        lo, hi = vv.min(0), vv.max(0)
+        ro = rng.uniform(lo - 0.5 * (hi - lo), hi + 0.5 * (hi - lo), size=(nr, 3)).astype(np.float32)
+        tgt = rng.uniform(lo, hi, size=(nr, 3)).astype(np.float32)
+        rd = tgt - ro
+        rd /= np.linalg.norm(rd, axis=1, keepdims=True)
+        rd = rd.astype(np.float32)
+        ro, rd = ro.reshape(-1), rd.reshape(-1)
+        bp, bt, bu, bv = bvh(ref, ob["verts"], ro, rd)
+        fp, ft, fu, fv = brute(ref, ob["verts"], ro, rd)
+        maxd = rng.uniform(0.1, 1.5, nr).astype(np.float32) * np.float32(np.linalg.norm(hi - lo))
+        occ = np.empty(nr, np.int32)
+        ref.ref_bvh_occluded(ob["verts"], ob["verts"].size // 9, nr, ro, rd, maxd, None, occ)
+        np.savez_compressed(os.path.join(HERE, f"rays_{key}.npz"), o=ro, d=rd, prim=bp, t=bt, u=bu, v=bv,
+                            brute_prim=fp, brute_t=ft, brute_u=fu, brute_v=fv, maxd=maxd, occluded=occ)
+        print(key, "tris", ob["verts"].size // 9, "primary hits", int((prim >= 0).sum()))
+    # Reference-estimator images of the Cornell asset.
+    sc = ref.rn_scene_load_obj(os.path.join(ASSETS, "cornell-box.obj").encode(), 0.35)
+    imgs = {}
+    for (W, H, spp) in [(160, 120, 64)]:
+        buf = np.empty(W * H * 4, np.uint8)
+        t0 = time.time()
+        ok = ref.rn_render(sc, 3, W, H, C.c_float(4.0), spp, buf)
+        assert ok == 1
+        print(f"reference rn_render {W}x{H} spp={spp}: {time.time() - t0:.1f}s")
+        imgs[f"rgba_{W}x{H}_{spp}"] = buf.reshape(H, W, 4)
+    ref.rn_scene_free(sc)
+    np.savez_compressed(os.path.join(HERE, "modeR_cornell.npz"), **imgs)
+    rng = np.random.default_rng(7)
+    pts = rng.uniform(-20, 20, size=(1024, 3)).astype(np.float32)
+    out = np.empty(1024, np.float32)
+    ref.ref_fractal_noise(1024, pts.reshape(-1), 4, out)
+    np.savez_compressed(os.path.join(HERE, "noise.npz"), p=pts, fbm4=out)
+
+
+if __name__ == "__main__":
+    main()
