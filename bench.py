@@ -1,0 +1,335 @@
+#!/usr/bin/env python3
+"""Benchmark of the path-trace hot path on one MI355X.
+
+Headline workload: a 1,048,576-triangle displaced sphere in
+the Cornell box, 1920x1080, 64 spp, max depth 8, Russian roulette from depth 3;
+every step rebuilds the LBVH/BVH8 on the GPU and renders the frame. `value` is
+the wavefront integrator's throughput in Mrays/s (closest-hit + shadow
+queries / wall time of the step, build included); the plain per-pixel
+megakernel is timed over the same number of steps beside it. Both run the same
+estimator with the same Philox streams, so their relative RMSE against the
+FP64 65k-spp oracle subset is equal by construction (reported).
+
+Also reported: the adjoint (config 3: 99,372-triangle sphere + Cornell walls,
+512x512, 32 spp, depth 6) ms/iteration, its ratio to the forward pass and its
+gradient error against central finite differences (h = 1e-3, 64 random
+triangles); the CPU oracle (FP32 port of the estimator) timed on a bounded
+pixel sample of the same workload on this host's cores; and, when present,
+the compiled reference's rn_render(RN_PATHTRACED) beside ours.
+
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Mrays/s @1M-tri 1080p, batched vs plain at equal RMSE; adjoint ms/iter"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_config1():
+    from rednoise_amd import scenes
+    sd = scenes.config1_scene(seed=0)
+    W, H, spp, depth, rr, eps = 1920, 1080, 64, 8, 3, 0.03
+    return sd, W, H, spp, depth, rr, eps
+
+
+def render_device(rn, lib, sc, cam, p, out_ptr):
+    if lib.rn_render_ex_device(sc.handle, C.byref(cam), C.byref(p), None, None, C.c_void_p(out_ptr), None) != 1:
+        raise RuntimeError(lib.rn_last_error().decode())
+    return rn.last_stats()
+
+
+def timed_steps(rn, lib, sc, cam, p, out_ptr, steps, warmup, torch):
+    """Each step: rebuild the BVH on the GPU, render the frame."""
+    for _ in range(warmup):
+        sc.build(p.bvh_width)
+        render_device(rn, lib, sc, cam, p, out_ptr)
+    torch.cuda.synchronize()
+    stats = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b = sc.build(p.bvh_width)
+        st = render_device(rn, lib, sc, cam, p, out_ptr)
+        st["build_ms"] = b["build_ms"]
+        stats.append(st)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    return dt, stats
+
+
+def rel_rmse(img, fixture):
+    pix = fixture["pixels"]
+    ref = fixture["values"]
+    got = img.reshape(-1, 3)[pix].astype(np.float64)
+    return float(np.sqrt(((got - ref) ** 2).sum() / (ref ** 2).sum()))
+
+
+def adjoint_bench(rn, lib, torch, n_fd=64):
+    from rednoise_amd import scenes
+    tgt, start = scenes.config3_scenes()
+    W = H = 512
+    spp, depth = 32, 6
+    cam = rn.Camera.make(**tgt.camera)
+    sc = rn.Scene.from_arrays(start.verts, start.albedo, start.emission)
+    sct = rn.Scene.from_arrays(tgt.verts, tgt.albedo, tgt.emission)
+    pt = rn.RenderParams.make(W, H, spp, depth, rr_depth=1 << 30, seed=1234, ray_eps=0.03)
+    target = sct.render_hdr(cam, pt).astype(np.float64)
+    p = rn.RenderParams.make(W, H, spp, depth, rr_depth=1 << 30, seed=0, ray_eps=0.03)
+    n = sc.triangle_count
+    img = sc.render_hdr(cam, p)
+    dL = (img.astype(np.float64) - target).astype(np.float32)  # L = 0.5 sum (I - T)^2
+    sc.adjoint(cam, p, dL)  # warm-up
+    iters = 3
+    t_fwd = t_adj = 0.0
+    for _ in range(iters):
+        t0 = time.perf_counter()
+        img = sc.render_hdr(cam, p)
+        t1 = time.perf_counter()
+        dL = (img.astype(np.float64) - target).astype(np.float32)
+        ga, ge = sc.adjoint(cam, p, dL)
+        t2 = time.perf_counter()
+        t_fwd += t1 - t0
+        t_adj += t2 - t1
+    fwd_ms, adj_ms = 1e3 * t_fwd / iters, 1e3 * t_adj / iters
+    st = rn.last_stats()
+    kern_adj = st["render_ms"]
+    sc.render_hdr(cam, p)
+    kern_fwd = rn.last_stats()["render_ms"]
+    # central differences on 64 random sphere triangles (albedo, one channel
+    # each), loss L = 0.5 sum (I - T)^2 evaluated in FP64 from the FP32 images.
+    rng = np.random.default_rng(64)
+    tris = rng.choice(np.arange(12, n), size=n_fd, replace=False)
+    chans = rng.integers(0, 3, size=n_fd)
+    alb = start.albedo.copy()
+    h = 1e-3
+    g_fd = np.zeros(n_fd)
+    g_ad = np.zeros(n_fd)
+    for k, (t, c) in enumerate(zip(tris, chans)):
+        a0 = alb[t, c]
+        alb[t, c] = a0 + h
+        sc.set_materials(albedo=alb)
+        ip = sc.render_hdr(cam, p).astype(np.float64)
+        hi = float(alb[t, c])
+        alb[t, c] = a0 - h
+        sc.set_materials(albedo=alb)
+        im = sc.render_hdr(cam, p).astype(np.float64)
+        lo = float(alb[t, c])
+        alb[t, c] = a0
+        dloss = 0.5 * float(((ip - im) * (ip + im - 2.0 * target)).sum())
+        g_fd[k] = dloss / (hi - lo)
+        g_ad[k] = ga[t, c]
+    sc.set_materials(albedo=alb)
+    err = float(np.linalg.norm(g_ad - g_fd) / max(1e-30, np.linalg.norm(g_fd)))
+    return {
+        "workload": f"config3: {n} tris (cube sphere m=91 + 12 Cornell), 512x512, 32 spp, depth 6, no roulette",
+        "ms_per_iter": round(fwd_ms + adj_ms, 3), "forward_ms": round(fwd_ms, 3), "adjoint_ms": round(adj_ms, 3),
+        "adjoint_over_forward": round(adj_ms / fwd_ms, 3),
+        "kernel_adjoint_over_forward": round(kern_adj / kern_fwd, 3),
+        "grad_rel_err_vs_fd": err, "fd_h": h, "fd_triangles": n_fd,
+        "target": "t_adjoint <= 3 t_forward and rel err <= 1e-2",
+    }
+
+
+def cpu_baseline(sd, W, H, spp, depth, rr, eps, budget_s=12.0):
+    """FP32 oracle (C port of the estimator, OpenMP) on a pixel sample of the
+    same workload; sample grows until ~budget_s of CPU work."""
+    from oracle import oracle as O
+    threads = os.cpu_count() or 1
+    t0 = time.perf_counter()
+    ps = O.PhysScene(sd.verts, sd.albedo, sd.emission, "f32")
+    prep = time.perf_counter() - t0
+    cam = O.camera_basis(sd.camera["eye"], sd.camera["target"], sd.camera["up"], sd.camera["vfov_deg"], W, H)
+    p = O.params(W, H, spp, depth, rr, 0.0, 0, 0, eps)
+    rng = np.random.default_rng(5)
+    npix = 64
+    total_rays, total_paths, total_t = 0, 0, 0.0
+    while total_t < budget_s and npix <= W * H:
+        pix = rng.choice(W * H, size=npix, replace=False).astype(np.int32)
+        t0 = time.perf_counter()
+        _, cnt = ps.render(cam, p, threads=threads, pixels=pix)
+        dt = time.perf_counter() - t0
+        total_rays += cnt.closest + cnt.shadow
+        total_paths += cnt.paths
+        total_t += dt
+        npix *= 2
+    ps.close()
+    return {"value": round(total_rays / total_t / 1e6, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": f"{total_paths // spp} random pixels x {spp} spp of the same 1080p frame (FP32 oracle port, "
+                      f"median-split BVH, OpenMP {threads} threads), {total_t:.1f}s render + {prep:.1f}s BVH build",
+            "samples_per_s": round(total_paths / total_t, 1)}
+
+
+def reference_modeR(rn, torch):
+    """The compiled reference rn_render(RN_PATHTRACED) (oracle/_ref) on the
+    Cornell asset beside this library's rn_render, same arguments."""
+    from oracle import oracle as O
+    ref = O.ref()
+    if ref is None:
+        return None
+    asset = os.path.join(ROOT, "tests", "golden", "assets", "cornell-box.obj")
+    W, H, spp = 320, 240, 16
+    sc = ref.rn_scene_load_obj(asset.encode(), C.c_float(0.35))
+    buf = np.empty(W * H * 4, np.uint8)
+    t0 = time.perf_counter()
+    ref.rn_render(sc, 3, W, H, C.c_float(4.0), spp, buf)
+    t_ref = time.perf_counter() - t0
+    ref.rn_scene_free(sc)
+    ours = rn.Scene.load_obj(asset, 0.35)
+    ours.render("pathtraced", W, H, 4.0, spp)
+    t0 = time.perf_counter()
+    ours.render("pathtraced", W, H, 4.0, spp)
+    t_gpu = time.perf_counter() - t0
+    return {"call": f"rn_render(RN_PATHTRACED) cornell-box.obj 320x240 spp={spp} cam_z=4",
+            "reference_cpu_s": round(t_ref, 4), "reference_threads": os.cpu_count(), "gpu_s": round(t_gpu, 5),
+            "speedup": round(t_ref / t_gpu, 1)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--skip-extras", action="store_true", help="headline only (no adjoint / CPU / RMSE)")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}, WORLD_SIZE={world}); the multi-GPU "
+              "sharding of this path is design-only.", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    import rednoise_amd as rn
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    torch.cuda.set_device(0)
+    lib = rn._load_library()
+    t0 = time.perf_counter()
+    sd, W, H, spp, depth, rr, eps = make_config1()
+    log(f"scene: {sd.notes} ({time.perf_counter() - t0:.1f}s)")
+    sc = rn.Scene.from_arrays(sd.verts, sd.albedo, sd.emission)
+    cam = rn.Camera.make(**sd.camera)
+    out = torch.empty(W * H * 3, dtype=torch.float32, device="cuda")
+    pw = rn.RenderParams.make(W, H, spp, depth, rr, seed=0, ray_eps=eps, integrator=rn.INTEGRATOR_WAVEFRONT,
+                              flags=rn.FLAG_TIME_STAGES)
+    pm = rn.RenderParams.make(W, H, spp, depth, rr, seed=0, ray_eps=eps, integrator=rn.INTEGRATOR_MEGAKERNEL)
+    sc.build(8)
+
+    # ---- timed: wavefront (value) and plain megakernel beside it
+    dt_w, st_w = timed_steps(rn, lib, sc, cam, pw, out.data_ptr(), args.steps, args.warmup, torch)
+    img_w = out.cpu().numpy()
+    log(f"wavefront: {1e3 * dt_w / args.steps:.1f} ms/step")
+    dt_m, st_m = timed_steps(rn, lib, sc, cam, pm, out.data_ptr(), args.steps, args.warmup, torch)
+    img_m = out.cpu().numpy()
+    log(f"megakernel: {1e3 * dt_m / args.steps:.1f} ms/step")
+    rays_w = sum(s["rays_closest"] + s["rays_shadow"] for s in st_w)
+    rays_m = sum(s["rays_closest"] + s["rays_shadow"] for s in st_m)
+    paths_w = sum(s["paths"] for s in st_w)
+    value = rays_w / dt_w / 1e6
+    plain = rays_m / dt_m / 1e6
+
+    # ---- traversal counts on the same BVH8 (instrumented megakernel)
+    pc = rn.RenderParams.make(W, H, spp, depth, rr, seed=0, ray_eps=eps, integrator=rn.INTEGRATOR_MEGAKERNEL,
+                              flags=rn.FLAG_COUNT_TRAVERSAL)
+    stc = render_device(rn, lib, sc, cam, pc, out.data_ptr())
+    nc = stc["nodes_closest"] / stc["rays_closest"]
+    tc = stc["tris_closest"] / stc["rays_closest"]
+    ns = stc["nodes_shadow"] / max(1, stc["rays_shadow"])
+    ts = stc["tris_shadow"] / max(1, stc["rays_shadow"])
+    b_closest = 48 + 80 * nc + 36 * tc
+    b_shadow = 36 + 80 * ns + 36 * ts
+    ext_ms = sum(s["extend_ms"] for s in st_w)
+    shd_ms = sum(s["shadow_ms"] for s in st_w)
+    launches = sum(s["iterations"] for s in st_w)
+    rc = sum(s["rays_closest"] for s in st_w)
+    rs = sum(s["rays_shadow"] for s in st_w)
+    if ext_ms >= shd_ms:
+        kname, kbytes, kms = "k_extend (wavefront closest-hit)", rc * b_closest, ext_ms
+    else:
+        kname, kbytes, kms = "k_shadow (wavefront any-hit)", rs * b_shadow, shd_ms
+    achieved = kbytes / (kms * 1e-3) / 1e9
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "traffic_config1.json")
+    if os.path.isfile(tf):
+        try:
+            traffic = json.load(open(tf)).get(kname.split()[0])
+        except Exception:
+            traffic = None
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic, "kernel": kname,
+                "avg_launch_ms": round(kms / max(1, launches), 4), "launches": launches,
+                "bytes_per_closest_ray": round(b_closest, 1), "bytes_per_shadow_ray": round(b_shadow, 1),
+                "nodes_per_closest_ray": round(nc, 3), "tris_per_closest_ray": round(tc, 3),
+                "nodes_per_shadow_ray": round(ns, 3), "tris_per_shadow_ray": round(ts, 3)}
+    megakernel_kernel_ms = sum(s["render_ms"] for s in st_m)
+    mk_bytes = sum(s["rays_closest"] for s in st_m) * b_closest + sum(s["rays_shadow"] for s in st_m) * b_shadow
+
+    result = {
+        "metric": METRIC, "value": round(value, 2), "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(1e3 * dt_w / args.steps, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded scene generator)",
+        "config": {"workload": "config1: 1,048,576-tri displaced sphere + 12 Cornell tris, 1920x1080, 64 spp, "
+                               "depth 8, RR from depth 3, LBVH+BVH8 rebuilt on the GPU every step, wavefront "
+                               "integrator", "triangles": sd.n, "width": W, "height": H, "spp": spp,
+                   "max_depth": depth},
+        "roofline": roofline,
+        "wavefront": {"mrays_s": round(value, 2), "ms_per_step": round(1e3 * dt_w / args.steps, 3),
+                      "samples_per_s": round(paths_w / dt_w, 1),
+                      "mrays_s_excl_build": round(rays_w / (dt_w - 1e-3 * sum(s["build_ms"] for s in st_w)) / 1e6,
+                                                  2),
+                      "extend_ms": round(ext_ms / args.steps, 2), "shade_ms": round(
+                          sum(s["shade_ms"] for s in st_w) / args.steps, 2),
+                      "shadow_ms": round(shd_ms / args.steps, 2),
+                      "iterations": st_w[0]["iterations"]},
+        "plain_megakernel": {"mrays_s": round(plain, 2), "ms_per_step": round(1e3 * dt_m / args.steps, 3),
+                             "kernel_ms": round(megakernel_kernel_ms / args.steps, 2),
+                             "kernel_achieved_gbs": round(mk_bytes / (megakernel_kernel_ms * 1e-3) / 1e9, 2)},
+        "wavefront_over_plain": round(value / plain, 3),
+        "build_ms": round(sum(s["build_ms"] for s in st_w) / args.steps, 3),
+        "bvh": {"nodes": st_w[0]["bvh_nodes"], "depth": st_w[0]["bvh_depth"]},
+        "rays_per_step": {"closest": st_w[0]["rays_closest"], "shadow": st_w[0]["rays_shadow"],
+                          "paths": st_w[0]["paths"]},
+    }
+    fx = os.path.join(ROOT, "tests", "golden", "oracle64k_config1.npz")
+    if os.path.isfile(fx):
+        f = np.load(fx)
+        result["equal_rmse"] = {"oracle": f"FP64 CPU oracle, {int(f['spp'])} spp, {len(f['pixels'])} pixels "
+                                          f"(one seeded pixel per {int(f['block'])}x{int(f['block'])} block)",
+                                "wavefront_rel_rmse": round(rel_rmse(img_w, f), 5),
+                                "plain_rel_rmse": round(rel_rmse(img_m, f), 5)}
+    if not args.skip_extras:
+        try:
+            result["adjoint"] = adjoint_bench(rn, lib, torch)
+            log(f"adjoint: {result['adjoint']}")
+        except Exception as e:  # reported, not fatal for the headline
+            result["adjoint"] = {"error": str(e)}
+        try:
+            r = reference_modeR(rn, torch)
+            if r:
+                result["reference_rn_render"] = r
+        except Exception as e:
+            result["reference_rn_render"] = {"error": str(e)}
+        result["cpu_baseline"] = cpu_baseline(sd, W, H, spp, depth, rr, eps)
+        result["cpu_baseline"]["gpu_over_cpu"] = round(value / result["cpu_baseline"]["value"], 1)
+    else:
+        result["cpu_baseline"] = None
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

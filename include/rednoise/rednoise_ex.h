@@ -84,7 +84,15 @@ typedef struct {
 	int queue_paths;  /* wavefront paths in flight; <= 0: chosen by the library */
 	int block_size;   /* threads per block for the trace kernels; <= 0: default */
 	int bvh_width;    /* 4 or 8 children per node; <= 0: 8 */
+	int flags;        /* RN_FLAG_* */
 } rn_render_params;
+
+/* rn_render_params.flags */
+enum {
+	RN_FLAG_COUNT_TRAVERSAL = 1, /* megakernel: also count BVH nodes / triangles per ray type (slower) */
+	RN_FLAG_TIME_STAGES = 2,     /* wavefront: time every stage launch with HIP events */
+	RN_FLAG_USE_GRAPH = 4        /* wavefront: replay the iteration from a captured hipGraph */
+};
 
 /* Physical ("Mode P") estimator: Lambertian albedo/pi, next-event estimation to
  * the emissive triangles (area-proportional choice), Russian roulette, black
@@ -145,6 +153,10 @@ typedef struct {
 	uint64_t trace_kernel_launches;
 	int bvh_nodes;
 	int bvh_depth;
+	uint64_t nodes_closest, tris_closest; /* with RN_FLAG_COUNT_TRAVERSAL */
+	uint64_t nodes_shadow, tris_shadow;
+	double extend_ms, shade_ms, shadow_ms; /* wavefront stage times (RN_FLAG_TIME_STAGES) */
+	uint64_t iterations;                  /* wavefront iterations */
 } rn_stats;
 
 int rn_get_last_stats(rn_stats *out);

@@ -30,6 +30,9 @@ __version__ = "0.1.2"
 
 INTEGRATOR_MEGAKERNEL = 0
 INTEGRATOR_WAVEFRONT = 1
+FLAG_COUNT_TRAVERSAL = 1
+FLAG_TIME_STAGES = 2
+FLAG_USE_GRAPH = 4
 QUERY_CLOSEST = 0
 QUERY_ANY = 1
 TRI_WATERTIGHT = 0
@@ -84,15 +87,15 @@ class RenderParams(C.Structure):
     _fields_ = [
         ("width", C.c_int), ("height", C.c_int), ("spp", C.c_int), ("max_depth", C.c_int), ("rr_depth", C.c_int),
         ("rr_prob", C.c_float), ("seed", C.c_uint32), ("ray_eps", C.c_float), ("integrator", C.c_int),
-        ("queue_paths", C.c_int), ("block_size", C.c_int), ("bvh_width", C.c_int),
+        ("queue_paths", C.c_int), ("block_size", C.c_int), ("bvh_width", C.c_int), ("flags", C.c_int),
     ]
 
     @classmethod
     def make(cls, width, height, spp, max_depth, rr_depth=1 << 30, rr_prob=0.0, seed=0, ray_eps=0.0,
-             integrator=INTEGRATOR_MEGAKERNEL, queue_paths=0, block_size=0, bvh_width=8) -> "RenderParams":
+             integrator=INTEGRATOR_MEGAKERNEL, queue_paths=0, block_size=0, bvh_width=8, flags=0) -> "RenderParams":
         return cls(int(width), int(height), int(spp), int(max_depth), int(min(rr_depth, 1 << 30)), float(rr_prob),
                    int(seed) & 0xFFFFFFFF, float(ray_eps), int(integrator), int(queue_paths), int(block_size),
-                   int(bvh_width))
+                   int(bvh_width), int(flags))
 
 
 class BatchItem(C.Structure):
@@ -113,7 +116,9 @@ class Stats(C.Structure):
         ("build_ms", C.c_double), ("render_ms", C.c_double), ("total_ms", C.c_double),
         ("rays_closest", C.c_uint64), ("rays_shadow", C.c_uint64), ("paths", C.c_uint64),
         ("trace_kernel_ms", C.c_double), ("trace_kernel_launches", C.c_uint64), ("bvh_nodes", C.c_int),
-        ("bvh_depth", C.c_int),
+        ("bvh_depth", C.c_int), ("nodes_closest", C.c_uint64), ("tris_closest", C.c_uint64),
+        ("nodes_shadow", C.c_uint64), ("tris_shadow", C.c_uint64), ("extend_ms", C.c_double),
+        ("shade_ms", C.c_double), ("shadow_ms", C.c_double), ("iterations", C.c_uint64),
     ]
 
     def as_dict(self) -> dict:

@@ -554,9 +554,12 @@ int rn_save_png(const char *path, int width, int height, const unsigned char *rg
 	RN_CATCH(0)
 }
 
+// Device-pointer entry points run on the caller's stream (NULL = the legacy
+// default stream, which orders them with torch's default stream); host-pointer
+// entry points run on the library's own stream.
 static int render_common(const rn_scene *scene, const rn_camera *camera, const rn_render_params *params,
                          const float *d_albedo, const float *d_emission, float *d_out, hipStream_t stream,
-                         float *h_out) {
+                         bool caller_stream, float *h_out) {
 	if (!scene || !camera || !params)
 		throw Error("NULL scene, camera or params");
 	auto t0 = std::chrono::steady_clock::now();
@@ -566,7 +569,7 @@ static int render_common(const rn_scene *scene, const rn_camera *camera, const r
 	std::lock_guard<std::mutex> sl(sc->mu);
 	DevParams p = make_params(*params, sc, 0);
 	DevCamera cam = make_camera(*camera, p.width, p.height);
-	hipStream_t s = stream ? stream : c.stream;
+	hipStream_t s = caller_stream ? stream : c.stream;
 	bool had = sc->bvh_valid && sc->bvh.width == (params->bvh_width > 0 ? params->bvh_width : 8);
 	sc->ensure_bvh(params->bvh_width, s);
 	size_t npx = (size_t)p.width * p.height;
@@ -576,14 +579,17 @@ static int render_common(const rn_scene *scene, const rn_camera *camera, const r
 	DevScene ds = sc->dev(d_albedo, d_emission);
 	RN_HIP(hipEventRecord(c.ev0, s));
 	WaveStats ws{};
-	if (params->integrator == RN_INTEGRATOR_WAVEFRONT) {
+	const bool wave = params->integrator == RN_INTEGRATOR_WAVEFRONT;
+	if (wave) {
 		WaveItem item{ds, cam, p, out};
 		WaveConfig cfg;
 		cfg.queue_paths = params->queue_paths;
 		cfg.block = params->block_size;
+		cfg.time_stages = (params->flags & RN_FLAG_TIME_STAGES) != 0;
+		cfg.use_graph = (params->flags & RN_FLAG_USE_GRAPH) != 0;
 		wavefront_render(&item, 1, cfg, cnt, s, ws);
 	} else {
-		launch_mega(ds, cam, p, out, cnt, params->block_size, s);
+		launch_mega(ds, cam, p, out, cnt, (params->flags & RN_FLAG_COUNT_TRAVERSAL) != 0, s);
 	}
 	RN_HIP(hipEventRecord(c.ev1, s));
 	DevCounters hc;
@@ -600,10 +606,18 @@ static int render_common(const rn_scene *scene, const rn_camera *camera, const r
 	t_stats.rays_closest = hc.closest;
 	t_stats.rays_shadow = hc.shadow;
 	t_stats.paths = hc.paths;
-	t_stats.trace_kernel_ms = params->integrator == RN_INTEGRATOR_WAVEFRONT ? ws.extend_ms : ms;
-	t_stats.trace_kernel_launches = params->integrator == RN_INTEGRATOR_WAVEFRONT ? ws.extend_launches : 1;
+	t_stats.trace_kernel_ms = wave ? std::max(ws.extend_ms, ws.shadow_ms) : ms;
+	t_stats.trace_kernel_launches = wave ? ws.extend_launches : 1;
 	t_stats.bvh_nodes = sc->bvh.n_nodes;
 	t_stats.bvh_depth = sc->bvh.depth;
+	t_stats.nodes_closest = hc.nodes_closest;
+	t_stats.tris_closest = hc.tris_closest;
+	t_stats.nodes_shadow = hc.nodes_shadow;
+	t_stats.tris_shadow = hc.tris_shadow;
+	t_stats.extend_ms = ws.extend_ms;
+	t_stats.shade_ms = ws.shade_ms;
+	t_stats.shadow_ms = ws.shadow_ms;
+	t_stats.iterations = ws.iterations;
 	return 1;
 }
 
@@ -612,7 +626,7 @@ int rn_render_ex(const rn_scene *scene, const rn_camera *camera, const rn_render
 	t_error.clear();
 	if (!hdr_rgb)
 		throw Error("hdr_rgb is NULL");
-	return render_common(scene, camera, params, nullptr, nullptr, nullptr, nullptr, hdr_rgb);
+	return render_common(scene, camera, params, nullptr, nullptr, nullptr, nullptr, false, hdr_rgb);
 	RN_CATCH(0)
 }
 
@@ -623,13 +637,14 @@ int rn_render_ex_device(const rn_scene *scene, const rn_camera *camera, const rn
 	if (!d_hdr_rgb)
 		throw Error("d_hdr_rgb is NULL");
 	return render_common(scene, camera, params, d_albedo, d_emission, d_hdr_rgb, static_cast<hipStream_t>(stream),
-	                     nullptr);
+	                     true, nullptr);
 	RN_CATCH(0)
 }
 
 static int adjoint_common(const rn_scene *scene, const rn_camera *camera, const rn_render_params *params,
                           const float *d_albedo, const float *d_emission, const float *d_dl, const float *h_dl,
-                          float *d_ga, float *d_ge, float *h_ga, float *h_ge, hipStream_t stream) {
+                          float *d_ga, float *d_ge, float *h_ga, float *h_ge, hipStream_t stream,
+                          bool caller_stream) {
 	if (!scene || !camera || !params)
 		throw Error("NULL scene, camera or params");
 	auto t0 = std::chrono::steady_clock::now();
@@ -639,7 +654,7 @@ static int adjoint_common(const rn_scene *scene, const rn_camera *camera, const 
 	std::lock_guard<std::mutex> sl(sc->mu);
 	DevParams p = make_params(*params, sc, 0);
 	DevCamera cam = make_camera(*camera, p.width, p.height);
-	hipStream_t s = stream ? stream : c.stream;
+	hipStream_t s = caller_stream ? stream : c.stream;
 	sc->ensure_bvh(params->bvh_width, s);
 	size_t npx = (size_t)p.width * p.height, n3 = 3 * (size_t)sc->h.n;
 	const float *dl = d_dl ? d_dl : upload(c.in0, h_dl, npx * 3, s);
@@ -677,7 +692,7 @@ int rn_adjoint(const rn_scene *scene, const rn_camera *camera, const rn_render_p
 	if (!dl_dimage)
 		throw Error("dl_dimage is NULL");
 	return adjoint_common(scene, camera, params, nullptr, nullptr, nullptr, dl_dimage, nullptr, nullptr, dl_dalbedo,
-	                      dl_demission, nullptr);
+	                      dl_demission, nullptr, false);
 	RN_CATCH(0)
 }
 
@@ -689,7 +704,7 @@ int rn_adjoint_device(const rn_scene *scene, const rn_camera *camera, const rn_r
 	if (!d_dl_dimage || !d_dl_dalbedo || !d_dl_demission)
 		throw Error("NULL device pointer");
 	return adjoint_common(scene, camera, params, d_albedo, d_emission, d_dl_dimage, nullptr, d_dl_dalbedo,
-	                      d_dl_demission, nullptr, nullptr, static_cast<hipStream_t>(stream));
+	                      d_dl_demission, nullptr, nullptr, static_cast<hipStream_t>(stream), true);
 	RN_CATCH(0)
 }
 
@@ -734,6 +749,7 @@ int rn_render_batch(const rn_batch_item *items, int count, const rn_batch_option
 	cfg.block = o.block_size;
 	cfg.use_graph = o.use_graph != 0;
 	cfg.iters_per_graph = o.iters_per_graph;
+	cfg.time_stages = !cfg.use_graph;
 	WaveStats ws{};
 	RN_HIP(hipEventRecord(c.ev0, c.stream));
 	wavefront_render(w.data(), count, cfg, cnt, c.stream, ws);
@@ -752,10 +768,14 @@ int rn_render_batch(const rn_batch_item *items, int count, const rn_batch_option
 	t_stats.rays_closest = hc.closest;
 	t_stats.rays_shadow = hc.shadow;
 	t_stats.paths = hc.paths;
-	t_stats.trace_kernel_ms = ws.extend_ms;
+	t_stats.trace_kernel_ms = std::max(ws.extend_ms, ws.shadow_ms);
 	t_stats.trace_kernel_launches = ws.extend_launches;
 	t_stats.bvh_nodes = nodes;
 	t_stats.bvh_depth = max_depth_bvh;
+	t_stats.extend_ms = ws.extend_ms;
+	t_stats.shade_ms = ws.shade_ms;
+	t_stats.shadow_ms = ws.shadow_ms;
+	t_stats.iterations = ws.iterations;
 	return 1;
 	RN_CATCH(0)
 }

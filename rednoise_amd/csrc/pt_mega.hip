@@ -41,10 +41,10 @@ RN_DEV void grad_add(float *glob, float *lds, const int *hot_slot, int prim, flo
 // One camera path of the physical estimator. With ADJ the path is a replay:
 // emission derivatives are pushed as they appear and the albedo derivatives
 // by the reverse sweep over the vertex records at the end.
-template <int LDS, int SPILL, bool ADJ>
+template <int LDS, int SPILL, bool ADJ, bool COUNT = false>
 RN_DEV float3 trace_sample(const DevScene &sc, const DevCamera &cam, const DevParams &p, int x, int y, uint32_t smp,
                            uint64_t *lds, int lane, unsigned long long &ncl, unsigned long long &nsh, float3 wgt,
-                           const GradSink *gs) {
+                           const GradSink *gs, uint32_t *cc = nullptr, uint32_t *cs = nullptr) {
 	const uint32_t pix = (uint32_t)(y * p.width + x);
 	u32x4 r = philox4x32_10(pix, smp, 0u, 0u, p.seed, p.scene_id);
 	float3 o = f3(cam.eye[0], cam.eye[1], cam.eye[2]);
@@ -54,7 +54,7 @@ RN_DEV float3 trace_sample(const DevScene &sc, const DevCamera &cam, const DevPa
 	int nrec = 0;
 	for (int depth = 0; depth < p.max_depth; depth++) {
 		ncl++;
-		Hit h = traverse<0, 0, LDS, SPILL>(sc.nodes, sc.tris, o, d, INFINITY, -1, lds, kMBlock, lane);
+		Hit h = traverse<0, 0, LDS, SPILL, COUNT>(sc.nodes, sc.tris, o, d, INFINITY, -1, lds, kMBlock, lane, cc);
 		if (h.prim < 0)
 			break;
 		float3 P = o + d * h.t;
@@ -73,7 +73,8 @@ RN_DEV float3 trace_sample(const DevScene &sc, const DevCamera &cam, const DevPa
 			NeeSample ns;
 			if (sample_light(sc, so, ng, u01(rn.x), u01(rn.y), u01(rn.z), ns)) {
 				nsh++;
-				Hit sh = traverse<1, 0, LDS, SPILL>(sc.nodes, sc.tris, so, ns.wi, ns.tmax, -1, lds, kMBlock, lane);
+				Hit sh = traverse<1, 0, LDS, SPILL, COUNT>(sc.nodes, sc.tris, so, ns.wi, ns.tmax, -1, lds, kMBlock,
+				                                           lane, cs);
 				if (sh.prim == 0) {
 					float3 le = load3(sc.emission, ns.light);
 					X = le * ns.Y;
@@ -118,7 +119,7 @@ RN_DEV float3 trace_sample(const DevScene &sc, const DevCamera &cam, const DevPa
 	return L;
 }
 
-template <int LDS, int SPILL>
+template <int LDS, int SPILL, bool COUNT>
 __global__ __launch_bounds__(kMBlock) void k_mega(DevScene sc, DevCamera cam, DevParams p, float *out,
                                                   DevCounters *cnt) {
 	__shared__ uint64_t stk[LDS * kMBlock];
@@ -128,11 +129,12 @@ __global__ __launch_bounds__(kMBlock) void k_mega(DevScene sc, DevCamera cam, De
 	int y = (int)(b / tiles_x) * kTileH + (threadIdx.x / kTileW);
 	bool valid = x < p.width && y < p.height;
 	unsigned long long ncl = 0, nsh = 0;
+	uint32_t cc[2] = {0u, 0u}, cs[2] = {0u, 0u};
 	if (valid) {
 		float3 sum = f3(0.f, 0.f, 0.f);
 		for (int s = 0; s < p.spp; s++)
-			sum = sum + trace_sample<LDS, SPILL, false>(sc, cam, p, x, y, (uint32_t)s, stk, threadIdx.x, ncl, nsh,
-			                                            sum, nullptr);
+			sum = sum + trace_sample<LDS, SPILL, false, COUNT>(sc, cam, p, x, y, (uint32_t)s, stk, threadIdx.x, ncl,
+			                                                   nsh, sum, nullptr, cc, cs);
 		float inv = 1.0f / (float)p.spp;
 		size_t o = 3 * ((size_t)y * p.width + x);
 		out[o] = sum.x * inv;
@@ -142,6 +144,12 @@ __global__ __launch_bounds__(kMBlock) void k_mega(DevScene sc, DevCamera cam, De
 	wave_add_u64(&cnt->closest, ncl);
 	wave_add_u64(&cnt->shadow, nsh);
 	wave_add_u64(&cnt->paths, valid ? (unsigned long long)p.spp : 0ull);
+	if (COUNT) {
+		wave_add_u64(&cnt->nodes_closest, cc[0]);
+		wave_add_u64(&cnt->tris_closest, cc[1]);
+		wave_add_u64(&cnt->nodes_shadow, cs[0]);
+		wave_add_u64(&cnt->tris_shadow, cs[1]);
+	}
 }
 
 template <int LDS, int SPILL>
@@ -185,12 +193,20 @@ unsigned mega_grid(const DevParams &p) {
 
 } // namespace
 
-void launch_mega(const DevScene &sc, const DevCamera &cam, const DevParams &p, float *out, DevCounters *cnt, int,
-                 hipStream_t s) {
-	if (sc.depth - 1 > kLdsDepth)
-		k_mega<kLdsDepth, kSpillDepth><<<mega_grid(p), kMBlock, 0, s>>>(sc, cam, p, out, cnt);
-	else
-		k_mega<kLdsDepth, 0><<<mega_grid(p), kMBlock, 0, s>>>(sc, cam, p, out, cnt);
+void launch_mega(const DevScene &sc, const DevCamera &cam, const DevParams &p, float *out, DevCounters *cnt,
+                 bool count_traversal, hipStream_t s) {
+	bool deep = sc.depth - 1 > kLdsDepth;
+	if (count_traversal) {
+		if (deep)
+			k_mega<kLdsDepth, kSpillDepth, true><<<mega_grid(p), kMBlock, 0, s>>>(sc, cam, p, out, cnt);
+		else
+			k_mega<kLdsDepth, 0, true><<<mega_grid(p), kMBlock, 0, s>>>(sc, cam, p, out, cnt);
+	} else {
+		if (deep)
+			k_mega<kLdsDepth, kSpillDepth, false><<<mega_grid(p), kMBlock, 0, s>>>(sc, cam, p, out, cnt);
+		else
+			k_mega<kLdsDepth, 0, false><<<mega_grid(p), kMBlock, 0, s>>>(sc, cam, p, out, cnt);
+	}
 	RN_HIP(hipGetLastError());
 }
 

@@ -258,9 +258,10 @@ struct Hit {
 // written out of bounds. QUERY 0 = closest hit, 1 = any hit (early exit).
 // TRI 0 = watertight, 1 = reference (compat) test. Triangles equal to
 // `ignore` (original id) are skipped.
-template <int QUERY, int TRI, int LDS_DEPTH, int SPILL>
+// With COUNT, cnt[0] += nodes visited and cnt[1] += triangles tested.
+template <int QUERY, int TRI, int LDS_DEPTH, int SPILL, bool COUNT = false>
 RN_DEV Hit traverse(const Bvh8Node *__restrict__ nodes, const float4 *__restrict__ tris, float3 o, float3 d,
-                    float tmax, int ignore, uint64_t *lds, int ldsStride, int lane) {
+                    float tmax, int ignore, uint64_t *lds, int ldsStride, int lane, uint32_t *cnt = nullptr) {
 	TRay r = make_tray(o, d);
 	WRay wr;
 	if (TRI == 0)
@@ -295,6 +296,8 @@ RN_DEV Hit traverse(const Bvh8Node *__restrict__ nodes, const float4 *__restrict
 			}
 			uint32_t w[20];
 			load_node(nodes, node, w);
+			if (COUNT)
+				cnt[0]++;
 			uint32_t ih;
 			intersect_children(w, r, h.t, ih, thits);
 			grp = mk_group(w[4], w[3] >> 24, ih);
@@ -308,6 +311,8 @@ RN_DEV Hit traverse(const Bvh8Node *__restrict__ nodes, const float4 *__restrict
 			int prim = __float_as_int(a.w);
 			if (prim == ignore)
 				continue;
+			if (COUNT)
+				cnt[1]++;
 			float tt, uu, vv;
 			bool ok;
 			if (TRI == 0)

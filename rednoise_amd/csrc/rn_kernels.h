@@ -17,6 +17,7 @@ constexpr int kMaxBvhDepth = kLdsDepth + kSpillDepth + 1;
 // Counters written by the integrators (device memory, 64-bit).
 struct DevCounters {
 	unsigned long long closest, shadow, paths, pad;
+	unsigned long long nodes_closest, tris_closest, nodes_shadow, tris_shadow;
 };
 
 void launch_trace_rays(const DevScene &sc, int n, const float *o, const float *d, const float *tmax,
@@ -26,8 +27,10 @@ void launch_primary_compat(const DevScene &sc, const CompatDev &cd, int W, int H
 void launch_philox(int n, const uint32_t *ctr, const uint32_t *key, uint32_t *out, hipStream_t s);
 
 // Plain per-pixel megakernel of the physical estimator: out = W*H*3 mean radiance.
+// With count_traversal the instrumented instance also sums BVH nodes visited
+// and triangles tested per ray type into the counters.
 void launch_mega(const DevScene &sc, const DevCamera &cam, const DevParams &p, float *out, DevCounters *cnt,
-                 int block, hipStream_t s);
+                 bool count_traversal, hipStream_t s);
 
 // Path-replay adjoint. hot_slot[prim] in [0, kHotSlots) selects an LDS
 // accumulator for heavily hit triangles, -1 otherwise.

@@ -2,12 +2,12 @@
 // "Megakernels Considered Harmful", HPG 2013), for one scene or a batch of
 // scenes of uneven size that share one path queue.
 //
-// Per iteration four persistent, grid-stride kernels run:
-//   extend  - appends freshly generated camera paths to the queue, then
-//             traces the closest hit of every queued path (SoA state);
-//   shade   - emission, NEE light sample, bounce and Russian roulette; pushes
-//             shadow rays and continuing paths with one wave64 ballot and one
-//             atomic per wave (compaction keeps the next queue dense);
+// Per iteration three persistent, grid-stride kernels run:
+//   extend  - appends freshly generated camera paths to the queue, traces the
+//             closest hit of every queued path (SoA state), then shades it:
+//             emission, NEE light sample, bounce and Russian roulette; shadow
+//             rays and continuing paths are pushed with one wave64 ballot and
+//             one atomic per wave (compaction keeps the next queue dense);
 //   shadow  - any-hit queries; unoccluded contributions are added to the
 //             frame buffer with float atomics;
 //   advance - one lane: rotates the queues and plans the next generation so
@@ -25,6 +25,10 @@ namespace rn {
 namespace {
 
 constexpr int kWBlock = 256;
+#ifndef RN_DYNAMIC_FETCH
+#define RN_DYNAMIC_FETCH 0
+#endif
+constexpr bool kUseDynamicFetch = RN_DYNAMIC_FETCH != 0;
 constexpr int kPixTile = 8; // 8x8 pixel tiles in generation order
 
 struct WScene {
@@ -40,11 +44,11 @@ struct WCtl {
 	unsigned cur_count, gen_count, next_count, sh_count;
 	unsigned long long gen_base, work_next, work_total;
 	unsigned done, iter;
+	unsigned sh_fetch; // dynamic-fetch cursor of the shadow queue
 };
 
 struct WQ {
 	float4 *o[2], *d[2], *b[2];
-	float4 *hit;
 	float4 *so, *sd, *sc;
 	WCtl *ctl;
 	const WScene *scenes;
@@ -65,6 +69,7 @@ __global__ void k_init(WCtl *ctl, unsigned long long total) {
 	ctl->work_total = total;
 	ctl->done = 0;
 	ctl->iter = 0;
+	ctl->sh_fetch = 0;
 }
 
 // Rotate queues and plan the next generation.
@@ -74,6 +79,7 @@ __global__ void k_advance(WQ q, int first) {
 		c->cur_count = c->next_count;
 	c->next_count = 0;
 	c->sh_count = 0;
+	c->sh_fetch = 0;
 	unsigned long long remaining = c->work_total - c->work_next;
 	unsigned long long room = (unsigned long long)(q.capacity - c->cur_count);
 	unsigned gen = (unsigned)(remaining < room ? remaining : room);
@@ -99,65 +105,6 @@ RN_DEV int find_scene(const WScene *sc, int n, unsigned long long g) {
 // meta word: scene index (low 16 bits) | depth << 16; depth 255 = dead slot.
 constexpr uint32_t kDead = 255u;
 
-template <int LDS, int SPILL>
-__global__ __launch_bounds__(kWBlock) void k_extend(WQ q, int cur) {
-	__shared__ uint64_t stk[LDS * kWBlock];
-	const WCtl *c = q.ctl;
-	const unsigned ncur = c->cur_count, n = ncur + c->gen_count;
-	const unsigned long long gbase = c->gen_base;
-	float4 *O = q.o[cur], *D = q.d[cur], *B = q.b[cur];
-	unsigned long long ncl = 0, npaths = 0;
-	for (unsigned base = blockIdx.x * kWBlock; base < n; base += gridDim.x * kWBlock) {
-		unsigned i = base + threadIdx.x;
-		if (i >= n)
-			continue;
-		float4 o4, d4, b4;
-		if (i >= ncur) { // new camera path
-			unsigned long long g = gbase + (i - ncur);
-			int si = find_scene(q.scenes, q.nscenes, g);
-			const WScene &ws = q.scenes[si];
-			unsigned long long local = g - ws.first;
-			uint32_t smp = (uint32_t)(local / (unsigned long long)ws.npix_pad);
-			uint32_t pp = (uint32_t)(local % (unsigned long long)ws.npix_pad);
-			uint32_t tile = pp >> 6, t = pp & 63u;
-			int x = (int)(tile % (uint32_t)ws.tiles_x) * kPixTile + (int)(t & 7u);
-			int y = (int)(tile / (uint32_t)ws.tiles_x) * kPixTile + (int)(t >> 3);
-			uint32_t pix = (uint32_t)(y * ws.p.width + x);
-			if (x < ws.p.width && y < ws.p.height) {
-				u32x4 r = philox4x32_10(pix, smp, 0u, 0u, ws.p.seed, ws.p.scene_id);
-				float3 d = camera_dir(ws.cam, x, y, ws.p.width, ws.p.height, u01(r.x), u01(r.y));
-				o4 = make_float4(ws.cam.eye[0], ws.cam.eye[1], ws.cam.eye[2], __uint_as_float(pix));
-				d4 = make_float4(d.x, d.y, d.z, __uint_as_float(smp));
-				b4 = make_float4(1.f, 1.f, 1.f, __uint_as_float((uint32_t)si));
-				npaths++;
-			} else {
-				o4 = make_float4(0.f, 0.f, 0.f, 0.f);
-				d4 = make_float4(0.f, 0.f, 1.f, 0.f);
-				b4 = make_float4(0.f, 0.f, 0.f, __uint_as_float((uint32_t)si | (kDead << 16)));
-			}
-			O[i] = o4;
-			D[i] = d4;
-			B[i] = b4;
-		} else {
-			o4 = O[i];
-			d4 = D[i];
-			b4 = B[i];
-		}
-		uint32_t meta = __float_as_uint(b4.w);
-		float4 hit = make_float4(0.f, __int_as_float(-1), 0.f, 0.f);
-		if ((meta >> 16) != kDead) {
-			const DevScene &sc = q.scenes[meta & 0xFFFFu].sc;
-			Hit h = traverse<0, 0, LDS, SPILL>(sc.nodes, sc.tris, xyz(o4), xyz(d4), INFINITY, -1, stk, kWBlock,
-			                                   threadIdx.x);
-			hit = make_float4(h.t, __int_as_float(h.prim), __uint_as_float(h.leaf), 0.f);
-			ncl++;
-		}
-		q.hit[i] = hit;
-	}
-	wave_add_u64(&q.cnt->closest, ncl);
-	wave_add_u64(&q.cnt->paths, npaths);
-}
-
 // Push with one atomic per wave: returns this lane's slot (valid when push).
 RN_DEV unsigned wave_push(unsigned *counter, bool push) {
 	uint64_t m = __ballot(push);
@@ -173,72 +120,113 @@ RN_DEV unsigned wave_push(unsigned *counter, bool push) {
 	return base + (unsigned)__popcll(m & lt);
 }
 
-__global__ __launch_bounds__(kWBlock) void k_shade(WQ q, int cur) {
+// Extend + shade: generate the new camera paths appended to the queue, trace
+// every queued path's closest hit, then (all lanes of the wave together)
+// add emission, draw the NEE light sample, the bounce and the roulette, and
+// push the shadow ray and the continuing path.
+template <int LDS, int SPILL>
+__global__ __launch_bounds__(kWBlock) void k_extend(WQ q, int cur) {
+	__shared__ uint64_t stk[LDS * kWBlock];
 	WCtl *c = q.ctl;
-	const unsigned n = c->cur_count + c->gen_count;
-	const float4 *O = q.o[cur], *D = q.d[cur], *B = q.b[cur];
+	const unsigned ncur = c->cur_count, n = ncur + c->gen_count;
+	const unsigned long long gbase = c->gen_base;
+	float4 *O = q.o[cur], *D = q.d[cur], *B = q.b[cur];
 	float4 *NO = q.o[cur ^ 1], *ND = q.d[cur ^ 1], *NB = q.b[cur ^ 1];
+	unsigned long long ncl = 0, npaths = 0;
 	for (unsigned base = blockIdx.x * kWBlock; base < n; base += gridDim.x * kWBlock) {
-		unsigned i = base + threadIdx.x;
-		bool valid = i < n;
-		bool push_sh = false, push_next = false;
-		float4 o4 = make_float4(0.f, 0.f, 0.f, 0.f), hit = make_float4(0.f, __int_as_float(-1), 0.f, 0.f);
-		float4 d4 = o4, b4 = o4;
-		float3 so = f3(0.f, 0.f, 0.f), wo = so, beta = so, contrib = so;
-		float3 shw = so;
-		float shmax = 0.f;
+		const unsigned i = base + threadIdx.x;
+		float4 o4 = make_float4(0.f, 0.f, 0.f, 0.f), d4 = o4, b4 = make_float4(0.f, 0.f, 0.f, 0.f);
 		uint32_t meta = kDead << 16;
-		if (valid) {
-			o4 = O[i];
-			d4 = D[i];
-			b4 = B[i];
-			hit = q.hit[i];
-			meta = __float_as_uint(b4.w);
+		if (i < n) {
+			if (i >= ncur) { // new camera path
+				unsigned long long g = gbase + (i - ncur);
+				int si = find_scene(q.scenes, q.nscenes, g);
+				const WScene &ws = q.scenes[si];
+				unsigned long long local = g - ws.first;
+				uint32_t smp = (uint32_t)(local / (unsigned long long)ws.npix_pad);
+				uint32_t pp = (uint32_t)(local % (unsigned long long)ws.npix_pad);
+				uint32_t tile = pp >> 6, t = pp & 63u;
+				int x = (int)(tile % (uint32_t)ws.tiles_x) * kPixTile + (int)(t & 7u);
+				int y = (int)(tile / (uint32_t)ws.tiles_x) * kPixTile + (int)(t >> 3);
+				uint32_t pix = (uint32_t)(y * ws.p.width + x);
+				meta = (uint32_t)si | (kDead << 16);
+				if (x < ws.p.width && y < ws.p.height) {
+					u32x4 r = philox4x32_10(pix, smp, 0u, 0u, ws.p.seed, ws.p.scene_id);
+					float3 d = camera_dir(ws.cam, x, y, ws.p.width, ws.p.height, u01(r.x), u01(r.y));
+					o4 = make_float4(ws.cam.eye[0], ws.cam.eye[1], ws.cam.eye[2], __uint_as_float(pix));
+					d4 = make_float4(d.x, d.y, d.z, __uint_as_float(smp));
+					b4 = make_float4(1.f, 1.f, 1.f, 0.f);
+					meta = (uint32_t)si;
+					npaths++;
+				}
+			} else {
+				o4 = O[i];
+				d4 = D[i];
+				b4 = B[i];
+				meta = __float_as_uint(b4.w);
+			}
 		}
-		int prim = __float_as_int(hit.y);
-		uint32_t depth = meta >> 16;
-		if (valid && depth != kDead && prim >= 0) {
-			const WScene &ws = q.scenes[meta & 0xFFFFu];
-			const DevScene &sc = ws.sc;
-			const DevParams &p = ws.p;
-			uint32_t pix = __float_as_uint(o4.w), smp = __float_as_uint(d4.w);
-			float3 o = xyz(o4), d = xyz(d4);
-			beta = xyz(b4);
-			float3 P = o + d * hit.x;
-			float3 ng = facing_normal(sc.tris, __float_as_uint(hit.z), d);
-			float *fb = ws.out + 3 * (size_t)pix;
-			if (depth == 0) {
-				float3 le = load3(sc.emission, prim);
-				if (le.x != 0.f || le.y != 0.f || le.z != 0.f) {
-					atomicAdd(fb, beta.x * le.x);
-					atomicAdd(fb + 1, beta.y * le.y);
-					atomicAdd(fb + 2, beta.z * le.z);
+		if (i < n && i >= ncur) { // publish the new path so it can be reloaded after traversal
+			O[i] = o4;
+			D[i] = d4;
+			B[i] = make_float4(b4.x, b4.y, b4.z, __uint_as_float(meta));
+		}
+		const uint32_t depth = meta >> 16;
+		bool push_sh = false, push_next = false;
+		float3 so = f3(0.f, 0.f, 0.f), wo = so, beta = so, contrib = so, shw = so;
+		float shmax = 0.f;
+		if (depth != kDead) {
+			const DevScene &sc0 = q.scenes[meta & 0xFFFFu].sc;
+			Hit h = traverse<0, 0, LDS, SPILL>(sc0.nodes, sc0.tris, xyz(o4), xyz(d4), INFINITY, -1, stk, kWBlock,
+			                                   threadIdx.x);
+			ncl++;
+			if (h.prim >= 0) {
+				// reload the state rather than keeping it live across traversal
+				o4 = O[i];
+				d4 = D[i];
+				b4 = B[i];
+				beta = xyz(b4);
+				const WScene &ws = q.scenes[meta & 0xFFFFu];
+				const DevScene &sc = ws.sc;
+				const DevParams &p = ws.p;
+				float3 o = xyz(o4), d = xyz(d4);
+				uint32_t pix = __float_as_uint(o4.w), smp = __float_as_uint(d4.w);
+				float3 P = o + d * h.t;
+				float3 ng = facing_normal(sc.tris, h.leaf, d);
+				if (depth == 0) {
+					float3 le = load3(sc.emission, h.prim);
+					if (le.x != 0.f || le.y != 0.f || le.z != 0.f) {
+						float *fb = ws.out + 3 * (size_t)pix;
+						atomicAdd(fb, beta.x * le.x);
+						atomicAdd(fb + 1, beta.y * le.y);
+						atomicAdd(fb + 2, beta.z * le.z);
+					}
 				}
-			}
-			float3 a = load3(sc.albedo, prim);
-			so = P + ng * p.ray_eps;
-			u32x4 rn = philox4x32_10(pix, smp, depth, 1u, p.seed, p.scene_id);
-			if (sc.nl > 0) {
-				NeeSample ns;
-				if (sample_light(sc, so, ng, u01(rn.x), u01(rn.y), u01(rn.z), ns)) {
-					float3 le = load3(sc.emission, ns.light);
-					contrib = beta * a * (le * ns.Y);
-					shw = ns.wi;
-					shmax = ns.tmax;
-					push_sh = true;
+				float3 a = load3(sc.albedo, h.prim);
+				so = P + ng * p.ray_eps;
+				u32x4 rn = philox4x32_10(pix, smp, depth, 1u, p.seed, p.scene_id);
+				if (sc.nl > 0) {
+					NeeSample ns;
+					if (sample_light(sc, so, ng, u01(rn.x), u01(rn.y), u01(rn.z), ns)) {
+						float3 le = load3(sc.emission, ns.light);
+						contrib = beta * a * (le * ns.Y);
+						shw = ns.wi;
+						shmax = ns.tmax;
+						push_sh = true;
+					}
 				}
-			}
-			if ((int)depth + 1 < p.max_depth) {
-				u32x4 rb = philox4x32_10(pix, smp, depth, 2u, p.seed, p.scene_id);
-				wo = cosine_dir(ng, u01(rb.x), u01(rb.y));
-				beta = beta * a;
-				push_next = true;
-				if ((int)depth >= p.rr_depth) {
-					float qs = rr_survival(p, beta);
-					if (!(u01(rn.w) < qs))
-						push_next = false;
-					else
-						beta = beta * (1.0f / qs);
+				if ((int)depth + 1 < p.max_depth) {
+					u32x4 rb = philox4x32_10(pix, smp, depth, 2u, p.seed, p.scene_id);
+					wo = cosine_dir(ng, u01(rb.x), u01(rb.y));
+					beta = beta * a;
+					push_next = true;
+					if ((int)depth >= p.rr_depth) {
+						float qs = rr_survival(p, beta);
+						if (!(u01(rn.w) < qs))
+							push_next = false;
+						else
+							beta = beta * (1.0f / qs);
+					}
 				}
 			}
 		}
@@ -255,6 +243,8 @@ __global__ __launch_bounds__(kWBlock) void k_shade(WQ q, int cur) {
 			NB[ns] = make_float4(beta.x, beta.y, beta.z, __uint_as_float((meta & 0xFFFFu) | ((depth + 1) << 16)));
 		}
 	}
+	wave_add_u64(&q.cnt->closest, ncl);
+	wave_add_u64(&q.cnt->paths, npaths);
 }
 
 template <int LDS, int SPILL>
@@ -281,6 +271,125 @@ __global__ __launch_bounds__(kWBlock) void k_shadow(WQ q) {
 	wave_add_u64(&q.cnt->shadow, nsh);
 }
 
+// Any-hit queue with dynamic fetch ("persistent while-while", Aila & Laine,
+// HPG 2009): each lane keeps its own traversal state; when at least
+// kFetchMin lanes of a wave have finished, the wave takes that many new rays
+// from the queue with one atomic, so lanes do not idle until the wave's
+// longest ray completes.
+constexpr int kFetchMin = 16;
+
+template <int LDS, int SPILL>
+__global__ __launch_bounds__(kWBlock) void k_shadow_df(WQ q) {
+	__shared__ uint64_t stk[LDS * kWBlock];
+	const unsigned n = q.ctl->sh_count;
+	unsigned *cursor = &q.ctl->sh_fetch;
+	const int lane = threadIdx.x & 63;
+	const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+	uint64_t *lds = stk;
+	int r = -1;
+	bool exhausted = false;
+	TRay ray;
+	WRay wr;
+	float tmax = 0.f;
+	const Bvh8Node *nodes = nullptr;
+	const float4 *tris = nullptr;
+	uint64_t grp = 0;
+	uint32_t tri_base = 0, thits = 0;
+	int sp = 0;
+	unsigned long long nsh = 0;
+	while (true) {
+		uint64_t idle = __ballot(r < 0);
+		if (!exhausted && __popcll(idle) >= (unsigned)kFetchMin) {
+			int leader = __ffsll((unsigned long long)idle) - 1;
+			unsigned base = 0;
+			if (lane == leader)
+				base = atomicAdd(cursor, (unsigned)__popcll(idle));
+			base = __shfl(base, leader);
+			if (base + (unsigned)__popcll(idle) >= n)
+				exhausted = true;
+			if (r < 0) {
+				unsigned idx = base + (unsigned)__popcll(idle & lt);
+				if (idx < n) {
+					r = (int)idx;
+					float4 o4 = q.so[idx], d4 = q.sd[idx];
+					const WScene &ws = q.scenes[__float_as_uint(q.sc[idx].w)];
+					nodes = ws.sc.nodes;
+					tris = ws.sc.tris;
+					ray = make_tray(xyz(o4), xyz(d4));
+					wr = make_wray(xyz(d4));
+					tmax = d4.w;
+					grp = mk_group(0u, 1u, 1u << ray.oct);
+					thits = 0;
+					sp = 0;
+					nsh++;
+				}
+			}
+		}
+		if (__ballot(r >= 0) == 0) {
+			if (exhausted || n == 0)
+				break;
+			continue;
+		}
+		if (r < 0)
+			continue;
+		bool done = false, occluded = false;
+		if (thits) {
+			while (thits) {
+				uint32_t k = __builtin_ctz(thits);
+				thits &= thits - 1u;
+				uint32_t ti = tri_base + k;
+				float4 a = tris[3 * ti], b4 = tris[3 * ti + 1], c4 = tris[3 * ti + 2];
+				float tt, uu, vv;
+				if (tri_watertight(wr, ray.o, xyz(a), xyz(b4), xyz(c4), tmax, tt, uu, vv)) {
+					occluded = true;
+					break;
+				}
+			}
+			if (occluded)
+				done = true;
+		} else {
+			uint32_t hits = (uint32_t)(grp >> 40);
+			if (hits) {
+				uint32_t b = __builtin_ctz(hits);
+				hits &= hits - 1u;
+				uint32_t slot = b ^ ray.oct;
+				uint32_t imask = (uint32_t)(grp >> 32) & 0xFFu;
+				uint32_t node = (uint32_t)grp + __builtin_popcount(imask & ((1u << slot) - 1u));
+				if (hits) {
+					uint64_t rest = (grp & 0xFFFFFFFFFFull) | ((uint64_t)hits << 40);
+					if (sp < LDS) {
+						lds[sp * kWBlock + threadIdx.x] = rest;
+						sp++;
+					}
+				}
+				uint32_t wv[20];
+				load_node(nodes, node, wv);
+				uint32_t ih;
+				intersect_children(wv, ray, tmax, ih, thits);
+				grp = mk_group(wv[4], wv[3] >> 24, ih);
+				tri_base = wv[5];
+			} else if (sp > 0) {
+				sp--;
+				grp = lds[sp * kWBlock + threadIdx.x];
+			} else {
+				done = true;
+			}
+		}
+		if (done) {
+			if (!occluded) {
+				float4 o4 = q.so[r], c4 = q.sc[r];
+				const WScene &ws = q.scenes[__float_as_uint(c4.w)];
+				float *fb = ws.out + 3 * (size_t)__float_as_uint(o4.w);
+				atomicAdd(fb, c4.x);
+				atomicAdd(fb + 1, c4.y);
+				atomicAdd(fb + 2, c4.z);
+			}
+			r = -1;
+		}
+	}
+	wave_add_u64(&q.cnt->shadow, nsh);
+}
+
 __global__ void k_resolve(float *out, size_t n, float scale) {
 	for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
 		out[i] *= scale;
@@ -288,8 +397,19 @@ __global__ void k_resolve(float *out, size_t n, float scale) {
 
 struct WaveWork {
 	DevBuf state, hit, shadow, ctl, scenes, host_flag;
-	std::vector<unsigned> dummy;
+	std::vector<hipEvent_t> events; // pool for per-stage timing
+	hipStream_t capture_stream = nullptr;
+	hipEvent_t join[2] = {nullptr, nullptr};
 };
+
+hipEvent_t pool_event(WaveWork &ww, size_t i) {
+	while (ww.events.size() <= i) {
+		hipEvent_t e;
+		RN_HIP(hipEventCreate(&e));
+		ww.events.push_back(e);
+	}
+	return ww.events[i];
+}
 
 WaveWork &work() {
 	static WaveWork w;
@@ -306,8 +426,32 @@ void fill_zero(void *p, size_t bytes, hipStream_t s) {
 	RN_HIP(hipGetLastError());
 }
 
+void wavefront_render_on(const WaveItem *items, int count, const WaveConfig &cfg, DevCounters *cnt, hipStream_t s,
+                         WaveStats &st);
+
+// The legacy default stream cannot be captured: a graph requested there runs
+// on a library stream ordered after, and before, the caller's work.
 void wavefront_render(const WaveItem *items, int count, const WaveConfig &cfg, DevCounters *cnt, hipStream_t s,
                       WaveStats &st) {
+	if (!(cfg.use_graph && s == nullptr)) {
+		wavefront_render_on(items, count, cfg, cnt, s, st);
+		return;
+	}
+	WaveWork &ww = work();
+	if (!ww.capture_stream) {
+		RN_HIP(hipStreamCreateWithFlags(&ww.capture_stream, hipStreamNonBlocking));
+		RN_HIP(hipEventCreateWithFlags(&ww.join[0], hipEventDisableTiming));
+		RN_HIP(hipEventCreateWithFlags(&ww.join[1], hipEventDisableTiming));
+	}
+	RN_HIP(hipEventRecord(ww.join[0], s));
+	RN_HIP(hipStreamWaitEvent(ww.capture_stream, ww.join[0], 0));
+	wavefront_render_on(items, count, cfg, cnt, ww.capture_stream, st);
+	RN_HIP(hipEventRecord(ww.join[1], ww.capture_stream));
+	RN_HIP(hipStreamWaitEvent(s, ww.join[1], 0));
+}
+
+void wavefront_render_on(const WaveItem *items, int count, const WaveConfig &cfg, DevCounters *cnt, hipStream_t s,
+                         WaveStats &st) {
 	if (count <= 0 || count > 65535)
 		throw Error("wavefront: batch size must be in [1, 65535]");
 	std::vector<WScene> hs((size_t)count);
@@ -342,7 +486,6 @@ void wavefront_render(const WaveItem *items, int count, const WaveConfig &cfg, D
 	q.o[1] = st4 + 3 * (size_t)cap;
 	q.d[1] = st4 + 4 * (size_t)cap;
 	q.b[1] = st4 + 5 * (size_t)cap;
-	q.hit = static_cast<float4 *>(ww.hit.reserve((size_t)cap * sizeof(float4)));
 	float4 *sh4 = static_cast<float4 *>(ww.shadow.reserve((size_t)cap * 3 * sizeof(float4)));
 	q.so = sh4;
 	q.sd = sh4 + cap;
@@ -363,16 +506,32 @@ void wavefront_render(const WaveItem *items, int count, const WaveConfig &cfg, D
 	grid = std::min<unsigned>(grid, (cap + kWBlock - 1) / kWBlock);
 	bool deep = max_depth - 1 > kLdsDepth;
 
+	// Stage timing: events bracket each launch; pairs are summed at the end.
+	const bool timed = cfg.time_stages && !cfg.use_graph;
+	std::vector<int> stage_of;
+	size_t nev = 0;
+	auto mark = [&](int stage, bool begin) {
+		if (!timed)
+			return;
+		RN_HIP(hipEventRecord(pool_event(ww, nev++), s));
+		if (begin)
+			stage_of.push_back(stage);
+	};
 	auto iteration = [&](int cur) {
+		mark(0, true);
 		if (deep)
 			k_extend<kLdsDepth, kSpillDepth><<<grid, kWBlock, 0, s>>>(q, cur);
 		else
 			k_extend<kLdsDepth, 0><<<grid, kWBlock, 0, s>>>(q, cur);
-		k_shade<<<grid, kWBlock, 0, s>>>(q, cur);
+		mark(0, false);
+		mark(2, true);
 		if (deep)
 			k_shadow<kLdsDepth, kSpillDepth><<<grid, kWBlock, 0, s>>>(q);
+		else if (kUseDynamicFetch)
+			k_shadow_df<kLdsDepth, 0><<<grid, kWBlock, 0, s>>>(q);
 		else
 			k_shadow<kLdsDepth, 0><<<grid, kWBlock, 0, s>>>(q);
+		mark(2, false);
 		k_advance<<<1, 1, 0, s>>>(q, 0);
 	};
 
@@ -418,6 +577,17 @@ void wavefront_render(const WaveItem *items, int count, const WaveConfig &cfg, D
 		}
 		st.iterations = iters;
 		st.extend_launches = iters;
+		if (timed) {
+			double acc[3] = {0, 0, 0};
+			for (size_t k = 0; k < stage_of.size(); k++) {
+				float ms = 0;
+				RN_HIP(hipEventElapsedTime(&ms, ww.events[2 * k], ww.events[2 * k + 1]));
+				acc[stage_of[k]] += ms;
+			}
+			st.extend_ms = acc[0];
+			st.shade_ms = acc[1];
+			st.shadow_ms = acc[2];
+		}
 	} catch (...) {
 		if (exec)
 			(void)hipGraphExecDestroy(exec);

@@ -1,0 +1,112 @@
+"""GPU path-replay adjoint: against the FP64 oracle adjoint on the same Philox
+streams, against central finite differences of the GPU forward pass, and
+through the torch.autograd.Function."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def setup(rn, oracle, sd, W, H, spp, depth, eps=0.05, seed=3):
+    sc = rn.Scene.from_arrays(sd.verts, sd.albedo, sd.emission)
+    cam = rn.Camera.make(**sd.camera)
+    p = rn.RenderParams.make(W, H, spp, depth, rr_depth=1 << 30, seed=seed, ray_eps=eps)
+    oc = oracle.camera_basis(sd.camera["eye"], sd.camera["target"], sd.camera["up"], sd.camera["vfov_deg"], W, H)
+    op = oracle.params(W, H, spp, depth, 100, 0.0, seed, 0, eps)
+    return sc, cam, p, oc, op
+
+
+def sphere_scene(n_target=5000):
+    from rednoise_amd import scenes
+    return scenes.sphere_scene_tris(n_target, seed=7)
+
+
+@pytest.mark.parametrize("which", ["cornell", "sphere"])
+def test_adjoint_matches_fp64_oracle(rn, oracle, which):
+    from rednoise_amd import scenes
+    sd = scenes.cornell_box() if which == "cornell" else sphere_scene()
+    W, H, spp, depth = 40, 30, 8, 3
+    sc, cam, p, oc, op = setup(rn, oracle, sd, W, H, spp, depth)
+    wgt = np.random.default_rng(1).normal(size=(H, W, 3)).astype(np.float32)
+    ga, ge = sc.adjoint(cam, p, wgt)
+    ps = oracle.PhysScene(sd.verts, sd.albedo, sd.emission, "f64")
+    oa, oe = ps.adjoint(oc, op, wgt)
+    ea = np.linalg.norm(ga - oa) / np.linalg.norm(oa)
+    ee = np.linalg.norm(ge - oe) / np.linalg.norm(oe)
+    assert ea < 2e-3 and ee < 2e-3, (ea, ee)
+
+
+def test_adjoint_vs_gpu_finite_differences(rn, oracle):
+    """Relative error <= 1e-2 against central differences (h = 1e-3) of the
+    GPU forward pass, loss L = 0.5 sum (I - T)^2."""
+    sd = sphere_scene(3000)
+    W, H, spp, depth = 96, 96, 16, 4
+    sc, cam, p, _, _ = setup(rn, oracle, sd, W, H, spp, depth, eps=0.03)
+    rng = np.random.default_rng(2)
+    target = sc.render_hdr(cam, p).astype(np.float64) * rng.uniform(0.5, 1.5, size=(H, W, 3))
+    img = sc.render_hdr(cam, p).astype(np.float64)
+    ga, ge = sc.adjoint(cam, p, (img - target).astype(np.float32))
+    alb = sd.albedo.copy()
+    tris = rng.choice(np.arange(12, sd.n), size=24, replace=False)
+    fd, ad = [], []
+    for t in tris:
+        c = int(rng.integers(0, 3))
+        a0 = alb[t, c]
+        alb[t, c] = a0 + np.float32(1e-3)
+        sc.set_materials(albedo=alb)
+        ip = sc.render_hdr(cam, p).astype(np.float64)
+        hi = float(alb[t, c])
+        alb[t, c] = a0 - np.float32(1e-3)
+        sc.set_materials(albedo=alb)
+        im = sc.render_hdr(cam, p).astype(np.float64)
+        lo = float(alb[t, c])
+        alb[t, c] = a0
+        fd.append(0.5 * float(((ip - im) * (ip + im - 2 * target)).sum()) / (hi - lo))
+        ad.append(float(ga[t, c]))
+    sc.set_materials(albedo=alb)
+    fd, ad = np.array(fd), np.array(ad)
+    err = np.linalg.norm(ad - fd) / np.linalg.norm(fd)
+    assert err < 1e-2, err
+    # the light's emission derivative as well
+    em = sd.emission.copy()
+    li = int(np.flatnonzero(em[:, 0] > 0)[0])
+    e0 = em[li, 1]
+    em[li, 1] = e0 + np.float32(1e-2)
+    sc.set_materials(emission=em)
+    ip = sc.render_hdr(cam, p).astype(np.float64)
+    hi = float(em[li, 1])
+    em[li, 1] = e0 - np.float32(1e-2)
+    sc.set_materials(emission=em)
+    im = sc.render_hdr(cam, p).astype(np.float64)
+    lo = float(em[li, 1])
+    fde = 0.5 * float(((ip - im) * (ip + im - 2 * target)).sum()) / (hi - lo)
+    assert abs(fde - ge[li, 1]) < 1e-2 * abs(fde)
+
+
+def test_torch_function(rn, oracle):
+    import torch
+
+    from rednoise_amd.differentiable import render
+    from rednoise_amd import scenes
+    sd = scenes.cornell_box()
+    W, H = 48, 32
+    sc, cam, p, _, _ = setup(rn, oracle, sd, W, H, 8, 3)
+    a = torch.tensor(sd.albedo, device="cuda", requires_grad=True)
+    e = torch.tensor(sd.emission, device="cuda", requires_grad=True)
+    img = render(sc, cam, p, a, e)
+    ref = sc.render_hdr(cam, p)
+    assert np.allclose(img.detach().cpu().numpy(), ref, rtol=1e-6, atol=1e-7)
+    w = torch.randn_like(img)
+    (img * w).sum().backward()
+    ga, ge = sc.adjoint(cam, p, w.cpu().numpy())
+    assert np.allclose(a.grad.cpu().numpy(), ga, rtol=1e-4, atol=1e-6)
+    assert np.allclose(e.grad.cpu().numpy(), ge, rtol=1e-4, atol=1e-6)
+    # gradient step lowers an L2 loss towards a target rendered with other albedos
+    tgt = torch.tensor(ref * 1.3, device="cuda")
+    a2 = torch.tensor(sd.albedo, device="cuda", requires_grad=True)
+    l0 = ((render(sc, cam, p, a2, e.detach()) - tgt) ** 2).sum()
+    l0.backward()
+    with torch.no_grad():
+        a3 = (a2 - 1e-3 * a2.grad / a2.grad.abs().max()).clamp(0, 1)
+    l1 = ((render(sc, cam, p, a3, e.detach()) - tgt) ** 2).sum()
+    assert float(l1) < float(l0)

@@ -1,0 +1,121 @@
+"""GPU rendering paths: the reference estimator against the compiled
+reference's images (3 standard errors per 16x16 tile), batching of uneven
+scenes in one wavefront launch, hipGraph replay, and the physical estimator
+against the white-furnace closed form."""
+import math
+import os
+
+import numpy as np
+import pytest
+
+from conftest import ASSETS, GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+
+def test_mode_r_statistical_parity_with_reference(rn):
+    """rn_render(RN_PATHTRACED) against renderPathTraced images of the compiled
+    reference: per 16x16 tile mean of the 8-bit channels within 3 standard
+    errors (SE from 12 independent GPU renders); at most 2% of tiles outside."""
+    g = np.load(os.path.join(GOLDEN, "modeR_cornell.npz"))
+    ref = g["rgba_160x120_64"].astype(np.float64)[..., :3]
+    H, W = ref.shape[:2]
+    sc = rn.Scene.load_obj(os.path.join(ASSETS, "cornell-box.obj"), 0.35)
+    K = 12
+    imgs = np.stack([sc.render_seeded(W, H, 4.0, 64, 1000 + k, 7).to_numpy()[..., :3].astype(np.float64)
+                     for k in range(K)])
+    T = 16
+
+    def tiles(a):
+        return a.reshape(*a.shape[:-3], H // T, T, W // T, T, 3).mean(axis=(-4, -2))
+
+    tm = tiles(imgs)                 # (K, ty, tx, 3)
+    mu = tm.mean(0)
+    sd = tm.std(0, ddof=1)
+    se = np.sqrt(sd ** 2 / K + sd ** 2) + 0.5 / T  # GPU mean + one reference render, + 8-bit floor
+    z = np.abs(mu - tiles(ref)) / se
+    frac = (z > 3).mean()
+    assert frac <= 0.02, (frac, z.max())
+    img0 = sc.render("pathtraced", W, H, 4.0, 64).to_numpy()
+    assert img0.shape == (H, W, 4) and (img0[..., 3] == 255).all()
+
+
+def test_white_furnace_gpu(rn):
+    from rednoise_amd import scenes
+    v, f = scenes.geodesic_sphere(4)
+    tv = v[f].astype(np.float32)
+    n = len(tv)
+    a, le = 0.6, 1.0
+    sc = rn.Scene.from_arrays(tv, np.full((n, 3), a, np.float32), np.full((n, 3), le, np.float32))
+    cam = rn.Camera.make((0.1, 0.05, -0.02), (1, 0.3, 0.2), (0, 1, 0), 60.0)
+    for D, rr, integ in ((6, 100, rn.INTEGRATOR_MEGAKERNEL), (16, 2, rn.INTEGRATOR_WAVEFRONT)):
+        p = rn.RenderParams.make(64, 64, 256, D, rr, seed=9, ray_eps=1e-5, integrator=integ)
+        img = sc.render_hdr(cam, p)[..., 0].astype(np.float64)
+        exp = le * (1 - a ** (D + 1)) / (1 - a)
+        se = img.std() / math.sqrt(img.size)
+        assert abs(img.mean() - exp) < max(4 * se, 2e-3 * exp), (D, img.mean(), exp)
+
+
+def _items(rn):
+    from rednoise_amd import scenes
+    out = []
+    for k, (tris, res, spp) in enumerate([(1500, (96, 64), 4), (20000, (64, 64), 16), (300, (40, 72), 8)]):
+        sd = scenes.sphere_scene_tris(tris, seed=50 + k)
+        sc = rn.Scene.from_arrays(sd.verts, sd.albedo, sd.emission)
+        cam = rn.Camera.make(**sd.camera)
+        p = rn.RenderParams.make(res[0], res[1], spp, 8, 3, seed=k, ray_eps=0.03)
+        out.append((sc, cam, p))
+    return out
+
+
+@pytest.mark.parametrize("graph", [0, 1])
+def test_batch_matches_single_scene_renders(rn, graph):
+    items = _items(rn)
+    opts = rn.BatchOptions(4096, 0, 8, graph, 4)
+    imgs = rn.render_batch(items, opts)
+    for (sc, cam, p), img in zip(items, imgs):
+        single = sc.render_hdr(cam, p)  # megakernel, same streams
+        assert np.allclose(img, single, rtol=1e-4, atol=1e-5)
+    st = rn.last_stats()
+    assert st["paths"] == sum(p.width * p.height * p.spp for _, _, p in items)
+    again = rn.render_batch(items, opts)  # the graph is rebuilt and replayed again
+    for a, b in zip(imgs, again):
+        assert np.allclose(a, b, rtol=1e-5, atol=1e-6)
+
+
+def test_graph_replays_match_eager(rn):
+    from rednoise_amd import scenes
+    sd = scenes.cornell_box()
+    sc = rn.Scene.from_arrays(sd.verts, sd.albedo, sd.emission)
+    cam = rn.Camera.make(**sd.camera)
+    base = rn.RenderParams.make(80, 60, 16, 6, 3, seed=4, ray_eps=0.05, integrator=rn.INTEGRATOR_WAVEFRONT,
+                                queue_paths=8192)
+    eager = sc.render_hdr(cam, base)
+    g = rn.RenderParams.make(80, 60, 16, 6, 3, seed=4, ray_eps=0.05, integrator=rn.INTEGRATOR_WAVEFRONT,
+                             queue_paths=8192, flags=rn.FLAG_USE_GRAPH)
+    for _ in range(3):
+        rep = sc.render_hdr(cam, g)
+        assert np.allclose(rep, eager, rtol=1e-5, atol=1e-6)
+        assert rn.last_stats()["iterations"] >= 8
+
+
+def test_bvh_width_4_matches_8(rn):
+    from rednoise_amd import scenes
+    sd = scenes.sphere_scene_tris(8000, seed=3)
+    sc = rn.Scene.from_arrays(sd.verts, sd.albedo, sd.emission)
+    cam = rn.Camera.make(**sd.camera)
+    p8 = rn.RenderParams.make(64, 48, 8, 6, 3, seed=1, ray_eps=0.03, bvh_width=8)
+    p4 = rn.RenderParams.make(64, 48, 8, 6, 3, seed=1, ray_eps=0.03, bvh_width=4)
+    a = sc.render_hdr(cam, p8)
+    b = sc.render_hdr(cam, p4)
+    assert np.allclose(a, b, rtol=1e-4, atol=1e-5)
+
+
+def test_single_triangle_scene(rn):
+    v = np.array([[[-10, -10, 5], [10, -10, 5], [0, 10, 5]]], np.float32)
+    sc = rn.Scene.from_arrays(v, np.full((1, 3), 0.5, np.float32), np.full((1, 3), 2.0, np.float32))
+    prim, t, u, w = sc.trace_rays(np.array([[0.2, 0.2, 0]], np.float32), np.array([[0, 0, 1]], np.float32))
+    assert prim[0] == 0 and abs(t[0] - 5) < 1e-5
+    cam = rn.Camera.make((0, 0, 0), (0, 0, 5), (0, 1, 0), 10.0)
+    img = sc.render_hdr(cam, rn.RenderParams.make(8, 8, 4, 3, seed=0, ray_eps=1e-3))
+    assert np.allclose(img, 2.0)  # every camera ray sees the emitter
