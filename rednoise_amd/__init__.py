@@ -149,6 +149,13 @@ def _load_library() -> C.CDLL:
     global _LIB
     if _LIB is not None:
         return _LIB
+    # PyTorch wheels ship their own libamdhip64.so under the same soname as
+    # the system one; whichever is loaded first serves the whole process.
+    # Loading torch first keeps a single HIP runtime when both are used.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     path = library_path()
     if not os.path.isfile(path):
         raise OSError(f"librednoise.so not found at {path!r}; build it with `python -c \"import __graft_entry__ as g; "

@@ -735,7 +735,7 @@ int rn_render_batch(const rn_batch_item *items, int count, const rn_batch_option
 		max_depth_bvh = std::max(max_depth_bvh, sc->bvh.depth);
 		nodes += sc->bvh.n_nodes;
 		w[(size_t)i].scene = sc->dev();
-		w[(size_t)i].params = make_params(it.params, sc, (uint32_t)i);
+		w[(size_t)i].params = make_params(it.params, sc, 0);
 		w[(size_t)i].cam = make_camera(it.camera, it.params.width, it.params.height);
 		offs[(size_t)i + 1] = offs[(size_t)i] + (size_t)it.params.width * it.params.height * 3;
 	}

@@ -1,8 +1,9 @@
 // GPU LBVH build: centroid bounds -> 30-bit Morton codes -> stable LSD radix
 // sort (wave64 ballot ranking) -> Karras (HPG 2012) binary radix tree ->
 // bottom-up AABB refit with agent-scope atomics -> top-down collapse into the
-// compressed 8-wide BVH of rn_types.h (greedy largest-area opening, leaves of
-// <= 3 triangles, octant-ordered child slots, 8-bit conservative boxes).
+// compressed 8-wide BVH of rn_types.h (greedy largest-area opening until nodes are
+// full, leaves of <= 3 triangles, octant-ordered child slots, 8-bit
+// conservative boxes).
 #include "rn_device.h"
 #include "rn_internal.h"
 
@@ -363,11 +364,14 @@ __global__ void k_collapse(int k0, int k1, int width, int *task, const float4 *b
 		e[0] = make_entry(left[b], n, box, range);
 		e[1] = make_entry(right[b], n, box, range);
 		m = 2;
+		// Open the largest-area internal entry until the node is full: large
+		// entries first (fewer levels), then small ones (tighter leaf boxes
+		// at no extra node visit).
 		while (m < width) {
 			int best = -1;
 			float ba = -1.0f;
 			for (int j = 0; j < m; j++) {
-				if (e[j].id >= 0 && e[j].last - e[j].first + 1 > kLeafMax) {
+				if (e[j].id >= 0) {
 					float a = half_area(e[j]);
 					if (a > ba) {
 						ba = a;

@@ -244,6 +244,15 @@ RN_DEV uint64_t mk_group(uint32_t base, uint32_t imask, uint32_t hits) {
 	return (uint64_t)base | ((uint64_t)imask << 32) | ((uint64_t)hits << 40);
 }
 
+// Triangle tests are postponed while fewer than 1/kPostponeDen of the
+// traversing lanes of a wave want them (0 = never). Measured on MI355X at
+// 1M triangles, postponing at 1/3, 1/5 or 1/8 was 4-8% slower than testing
+// triangles at once, so it is off by default.
+#ifndef RN_POSTPONE_DEN
+#define RN_POSTPONE_DEN 0
+#endif
+constexpr uint32_t kPostponeDen = RN_POSTPONE_DEN;
+
 struct Hit {
 	int prim;      // original triangle id, -1 = miss (any-hit: 1 = occluded)
 	uint32_t leaf; // leaf-ordered triangle index of the hit
@@ -276,6 +285,11 @@ RN_DEV Hit traverse(const Bvh8Node *__restrict__ nodes, const float4 *__restrict
 	int sp = 0;
 	uint64_t grp = mk_group(0u, 1u, 1u << (0u ^ r.oct));
 	uint32_t tri_base = 0, thits = 0;
+	// One postponed triangle group per lane (register-resident, so the stack
+	// bound is unchanged): when fewer than 1/kPostponeDen of the wave's
+	// traversing lanes want triangles, a lane that still has nodes to visit
+	// keeps its new leaf for later instead of idling the others.
+	uint32_t dbase = 0, dhits = 0;
 	while (true) {
 		uint32_t hits = (uint32_t)(grp >> 40);
 		if (hits) {
@@ -303,33 +317,50 @@ RN_DEV Hit traverse(const Bvh8Node *__restrict__ nodes, const float4 *__restrict
 			grp = mk_group(w[4], w[3] >> 24, ih);
 			tri_base = w[5];
 		}
-		while (thits) {
-			uint32_t k = __builtin_ctz(thits);
-			thits &= thits - 1u;
-			uint32_t ti = tri_base + k;
-			float4 a = tris[3 * ti], b4 = tris[3 * ti + 1], c4 = tris[3 * ti + 2];
-			int prim = __float_as_int(a.w);
-			if (prim == ignore)
-				continue;
-			if (COUNT)
-				cnt[1]++;
-			float tt, uu, vv;
-			bool ok;
-			if (TRI == 0)
-				ok = tri_watertight(wr, o, xyz(a), xyz(b4), xyz(c4), h.t, tt, uu, vv);
-			else
-				ok = tri_compat(o, d, xyz(a), xyz(b4), xyz(c4), tt, uu, vv) && tt < h.t;
-			if (ok) {
-				if (QUERY == 1) {
-					h.prim = 1;
-					return h;
+		const uint64_t active = __ballot(1);
+		const uint64_t want = __ballot((thits | dhits) != 0u);
+		bool defer = kPostponeDen > 0 && thits != 0u && dhits == 0u && (grp >> 40) != 0 &&
+		             (uint32_t)__popcll(want) * kPostponeDen < (uint32_t)__popcll(active);
+		if (defer) {
+			dbase = tri_base;
+			dhits = thits;
+			thits = 0;
+		} else if (thits | dhits) {
+			// run the current group, then the postponed one
+			for (int g = 0; g < 2; g++) {
+				uint32_t base = g == 0 ? tri_base : dbase;
+				uint32_t m = g == 0 ? thits : dhits;
+				while (m) {
+					uint32_t k = __builtin_ctz(m);
+					m &= m - 1u;
+					uint32_t ti = base + k;
+					float4 a = tris[3 * ti], b4 = tris[3 * ti + 1], c4 = tris[3 * ti + 2];
+					int prim = __float_as_int(a.w);
+					if (prim == ignore)
+						continue;
+					if (COUNT)
+						cnt[1]++;
+					float tt, uu, vv;
+					bool ok;
+					if (TRI == 0)
+						ok = tri_watertight(wr, o, xyz(a), xyz(b4), xyz(c4), h.t, tt, uu, vv);
+					else
+						ok = tri_compat(o, d, xyz(a), xyz(b4), xyz(c4), tt, uu, vv) && tt < h.t;
+					if (ok) {
+						if (QUERY == 1) {
+							h.prim = 1;
+							return h;
+						}
+						h.prim = prim;
+						h.leaf = ti;
+						h.t = tt;
+						h.u = uu;
+						h.v = vv;
+					}
 				}
-				h.prim = prim;
-				h.leaf = ti;
-				h.t = tt;
-				h.u = uu;
-				h.v = vv;
 			}
+			thits = 0;
+			dhits = 0;
 		}
 		if ((grp >> 40) == 0) {
 			if (sp == 0)

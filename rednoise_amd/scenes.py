@@ -225,35 +225,32 @@ def geodesic_sphere(levels: int):
 
 
 def cube_sphere(m: int):
-    """Cube sphere with an m x m grid per face: 12 m^2 triangles, vertices on
-    shared cube edges coincide exactly (watertight). Unit radius."""
-    g = np.linspace(-1.0, 1.0, m + 1)
-    verts, faces = [], []
-    key = {}
-    axes = [(0, 1, 2, 1.0), (0, 1, 2, -1.0), (1, 2, 0, 1.0), (1, 2, 0, -1.0), (2, 0, 1, 1.0), (2, 0, 1, -1.0)]
-    for (a, b, c, sgn) in axes:
-        idx = np.empty((m + 1, m + 1), dtype=np.int64)
-        for i in range(m + 1):
-            for j in range(m + 1):
-                p = [0.0, 0.0, 0.0]
-                p[a] = g[i]
-                p[b] = g[j]
-                p[c] = sgn
-                k = (round(p[0] * m), round(p[1] * m), round(p[2] * m))
-                if k not in key:
-                    key[k] = len(verts)
-                    verts.append(p)
-                idx[i, j] = key[k]
+    """Cube sphere with an m x m grid per face: 12 m^2 triangles. Vertices on
+    shared cube edges are merged by their integer lattice key, so the mesh is
+    watertight. Unit radius."""
+    ij = np.arange(m + 1)
+    I, J = np.meshgrid(ij, ij, indexing="ij")
+    keys, tris = [], []
+    axes = [(0, 1, 2, 1), (0, 1, 2, -1), (1, 2, 0, 1), (1, 2, 0, -1), (2, 0, 1, 1), (2, 0, 1, -1)]
+    for f, (a, b, c, sgn) in enumerate(axes):
+        lat = np.zeros((m + 1, m + 1, 3), np.int64)  # lattice coords in [-m, m]
+        lat[..., a] = 2 * I - m
+        lat[..., b] = 2 * J - m
+        lat[..., c] = sgn * m
+        keys.append(lat.reshape(-1, 3))
+        base = f * (m + 1) * (m + 1)
+        idx = base + (I * (m + 1) + J)
         q00, q10, q01, q11 = idx[:-1, :-1], idx[1:, :-1], idx[:-1, 1:], idx[1:, 1:]
         if sgn > 0:
-            faces.append(np.stack([q00, q10, q11], -1).reshape(-1, 3))
-            faces.append(np.stack([q00, q11, q01], -1).reshape(-1, 3))
+            tris += [np.stack([q00, q10, q11], -1).reshape(-1, 3), np.stack([q00, q11, q01], -1).reshape(-1, 3)]
         else:
-            faces.append(np.stack([q00, q11, q10], -1).reshape(-1, 3))
-            faces.append(np.stack([q00, q01, q11], -1).reshape(-1, 3))
-    v = np.asarray(verts, dtype=np.float64)
+            tris += [np.stack([q00, q11, q10], -1).reshape(-1, 3), np.stack([q00, q01, q11], -1).reshape(-1, 3)]
+    allk = np.concatenate(keys)
+    uniq, inv = np.unique(allk, axis=0, return_inverse=True)
+    v = uniq.astype(np.float64) / m
     v /= np.linalg.norm(v, axis=1, keepdims=True)
-    return v, np.concatenate(faces)
+    faces = inv.reshape(-1)[np.concatenate(tris)]
+    return v, faces
 
 
 def displace(unit_verts: np.ndarray, radius: float, amplitude: float, seed: int, octaves: int = 4,

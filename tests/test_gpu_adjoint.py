@@ -109,4 +109,4 @@ def test_torch_function(rn, oracle):
     with torch.no_grad():
         a3 = (a2 - 1e-3 * a2.grad / a2.grad.abs().max()).clamp(0, 1)
     l1 = ((render(sc, cam, p, a3, e.detach()) - tgt) ** 2).sum()
-    assert float(l1) < float(l0)
+    assert float(l1.detach()) < float(l0.detach())
