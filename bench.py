@@ -146,6 +146,51 @@ def adjoint_bench(rn, lib, torch, n_fd=64):
     }
 
 
+def batch_bench(rn, torch):
+    """Config 2: 64 seeded displaced spheres (1k-500k triangles, 256^2..1024^2,
+    4/16/64 spp, depth 8) in one set of persistent wavefront launches replayed
+    from a hipGraph, options picked by the autotuner; the plain megakernel
+    renders the same scenes one after another beside it."""
+    from rednoise_amd import autotune, scenes
+    t0 = time.perf_counter()
+    specs = scenes.config2_batch(64, seed=2)
+    items = []
+    for s in specs:
+        sc = rn.Scene.from_arrays(s.scene.verts, s.scene.albedo, s.scene.emission)
+        items.append((sc, rn.Camera.make(**s.scene.camera),
+                      rn.RenderParams.make(s.width, s.height, s.spp, s.max_depth, 3, seed=0, ray_eps=0.03)))
+    gen_s = time.perf_counter() - t0
+    opts, report = autotune.autotune_batch(items, use_cache=False)
+    rn.render_batch(items, opts)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    imgs = rn.render_batch(items, opts)
+    dt = time.perf_counter() - t0
+    st = rn.last_stats()
+    rays = st["rays_closest"] + st["rays_shadow"]
+    for sc, cam, p in items:  # the per-scene megakernel path runs on width-8 BVHs, built here
+        p.bvh_width = 8
+        sc.build(8)
+    for sc, cam, p in items[:2]:
+        sc.render_hdr(cam, p)
+    t0 = time.perf_counter()
+    plain_rays = 0
+    for sc, cam, p in items:
+        sc.render_hdr(cam, p)
+        s2 = rn.last_stats()
+        plain_rays += s2["rays_closest"] + s2["rays_shadow"]
+    dt_plain = time.perf_counter() - t0
+    return {"workload": f"config2: 64 scenes, {sum(sc.triangle_count for sc, _, _ in items):,} triangles, "
+                        f"{sum(p.width * p.height * p.spp for _, _, p in items) / 1e6:.1f}M paths, depth 8, "
+                        "one persistent wavefront, hipGraph replay",
+            "batched_mrays_s": round(rays / dt / 1e6, 2), "batched_s": round(dt, 3),
+            "plain_sequential_mrays_s": round(plain_rays / dt_plain / 1e6, 2), "plain_s": round(dt_plain, 3),
+            "batched_over_plain": round((rays / dt) / (plain_rays / dt_plain), 3),
+            "autotune": {"best": report["best"], "candidates": len(report["candidates"]),
+                         "trial_spp_fraction": report["trial_spp_fraction"]},
+            "scene_generation_s": round(gen_s, 1)}
+
+
 def cpu_baseline(sd, W, H, spp, depth, rr, eps, budget_s=12.0):
     """FP32 oracle (C port of the estimator, OpenMP) on a pixel sample of the
     same workload; sample grows until ~budget_s of CPU work."""
@@ -318,6 +363,11 @@ def main():
             log(f"adjoint: {result['adjoint']}")
         except Exception as e:  # reported, not fatal for the headline
             result["adjoint"] = {"error": str(e)}
+        try:
+            result["batch"] = batch_bench(rn, torch)
+            log(f"batch: {result['batch']}")
+        except Exception as e:
+            result["batch"] = {"error": str(e)}
         try:
             r = reference_modeR(rn, torch)
             if r:

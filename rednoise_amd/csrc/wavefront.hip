@@ -24,11 +24,6 @@ namespace rn {
 
 namespace {
 
-constexpr int kWBlock = 256;
-#ifndef RN_DYNAMIC_FETCH
-#define RN_DYNAMIC_FETCH 0
-#endif
-constexpr bool kUseDynamicFetch = RN_DYNAMIC_FETCH != 0;
 constexpr int kPixTile = 8; // 8x8 pixel tiles in generation order
 
 struct WScene {
@@ -44,7 +39,6 @@ struct WCtl {
 	unsigned cur_count, gen_count, next_count, sh_count;
 	unsigned long long gen_base, work_next, work_total;
 	unsigned done, iter;
-	unsigned sh_fetch; // dynamic-fetch cursor of the shadow queue
 };
 
 struct WQ {
@@ -69,7 +63,6 @@ __global__ void k_init(WCtl *ctl, unsigned long long total) {
 	ctl->work_total = total;
 	ctl->done = 0;
 	ctl->iter = 0;
-	ctl->sh_fetch = 0;
 }
 
 // Rotate queues and plan the next generation.
@@ -79,7 +72,6 @@ __global__ void k_advance(WQ q, int first) {
 		c->cur_count = c->next_count;
 	c->next_count = 0;
 	c->sh_count = 0;
-	c->sh_fetch = 0;
 	unsigned long long remaining = c->work_total - c->work_next;
 	unsigned long long room = (unsigned long long)(q.capacity - c->cur_count);
 	unsigned gen = (unsigned)(remaining < room ? remaining : room);
@@ -124,16 +116,16 @@ RN_DEV unsigned wave_push(unsigned *counter, bool push) {
 // every queued path's closest hit, then (all lanes of the wave together)
 // add emission, draw the NEE light sample, the bounce and the roulette, and
 // push the shadow ray and the continuing path.
-template <int LDS, int SPILL>
-__global__ __launch_bounds__(kWBlock) void k_extend(WQ q, int cur) {
-	__shared__ uint64_t stk[LDS * kWBlock];
+template <int LDS, int SPILL, int BS>
+__global__ __launch_bounds__(BS) void k_extend(WQ q, int cur) {
+	__shared__ uint64_t stk[LDS * BS];
 	WCtl *c = q.ctl;
 	const unsigned ncur = c->cur_count, n = ncur + c->gen_count;
 	const unsigned long long gbase = c->gen_base;
 	float4 *O = q.o[cur], *D = q.d[cur], *B = q.b[cur];
 	float4 *NO = q.o[cur ^ 1], *ND = q.d[cur ^ 1], *NB = q.b[cur ^ 1];
 	unsigned long long ncl = 0, npaths = 0;
-	for (unsigned base = blockIdx.x * kWBlock; base < n; base += gridDim.x * kWBlock) {
+	for (unsigned base = blockIdx.x * BS; base < n; base += gridDim.x * BS) {
 		const unsigned i = base + threadIdx.x;
 		float4 o4 = make_float4(0.f, 0.f, 0.f, 0.f), d4 = o4, b4 = make_float4(0.f, 0.f, 0.f, 0.f);
 		uint32_t meta = kDead << 16;
@@ -177,7 +169,7 @@ __global__ __launch_bounds__(kWBlock) void k_extend(WQ q, int cur) {
 		float shmax = 0.f;
 		if (depth != kDead) {
 			const DevScene &sc0 = q.scenes[meta & 0xFFFFu].sc;
-			Hit h = traverse<0, 0, LDS, SPILL>(sc0.nodes, sc0.tris, xyz(o4), xyz(d4), INFINITY, -1, stk, kWBlock,
+			Hit h = traverse<0, 0, LDS, SPILL>(sc0.nodes, sc0.tris, xyz(o4), xyz(d4), INFINITY, -1, stk, BS,
 			                                   threadIdx.x);
 			ncl++;
 			if (h.prim >= 0) {
@@ -247,18 +239,18 @@ __global__ __launch_bounds__(kWBlock) void k_extend(WQ q, int cur) {
 	wave_add_u64(&q.cnt->paths, npaths);
 }
 
-template <int LDS, int SPILL>
-__global__ __launch_bounds__(kWBlock) void k_shadow(WQ q) {
-	__shared__ uint64_t stk[LDS * kWBlock];
+template <int LDS, int SPILL, int BS>
+__global__ __launch_bounds__(BS) void k_shadow(WQ q) {
+	__shared__ uint64_t stk[LDS * BS];
 	const unsigned n = q.ctl->sh_count;
 	unsigned long long nsh = 0;
-	for (unsigned base = blockIdx.x * kWBlock; base < n; base += gridDim.x * kWBlock) {
+	for (unsigned base = blockIdx.x * BS; base < n; base += gridDim.x * BS) {
 		unsigned i = base + threadIdx.x;
 		if (i >= n)
 			continue;
 		float4 o4 = q.so[i], d4 = q.sd[i], c4 = q.sc[i];
 		const WScene &ws = q.scenes[__float_as_uint(c4.w)];
-		Hit h = traverse<1, 0, LDS, SPILL>(ws.sc.nodes, ws.sc.tris, xyz(o4), xyz(d4), d4.w, -1, stk, kWBlock,
+		Hit h = traverse<1, 0, LDS, SPILL>(ws.sc.nodes, ws.sc.tris, xyz(o4), xyz(d4), d4.w, -1, stk, BS,
 		                                   threadIdx.x);
 		nsh++;
 		if (h.prim == 0) {
@@ -266,125 +258,6 @@ __global__ __launch_bounds__(kWBlock) void k_shadow(WQ q) {
 			atomicAdd(fb, c4.x);
 			atomicAdd(fb + 1, c4.y);
 			atomicAdd(fb + 2, c4.z);
-		}
-	}
-	wave_add_u64(&q.cnt->shadow, nsh);
-}
-
-// Any-hit queue with dynamic fetch ("persistent while-while", Aila & Laine,
-// HPG 2009): each lane keeps its own traversal state; when at least
-// kFetchMin lanes of a wave have finished, the wave takes that many new rays
-// from the queue with one atomic, so lanes do not idle until the wave's
-// longest ray completes.
-constexpr int kFetchMin = 16;
-
-template <int LDS, int SPILL>
-__global__ __launch_bounds__(kWBlock) void k_shadow_df(WQ q) {
-	__shared__ uint64_t stk[LDS * kWBlock];
-	const unsigned n = q.ctl->sh_count;
-	unsigned *cursor = &q.ctl->sh_fetch;
-	const int lane = threadIdx.x & 63;
-	const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-	uint64_t *lds = stk;
-	int r = -1;
-	bool exhausted = false;
-	TRay ray;
-	WRay wr;
-	float tmax = 0.f;
-	const Bvh8Node *nodes = nullptr;
-	const float4 *tris = nullptr;
-	uint64_t grp = 0;
-	uint32_t tri_base = 0, thits = 0;
-	int sp = 0;
-	unsigned long long nsh = 0;
-	while (true) {
-		uint64_t idle = __ballot(r < 0);
-		if (!exhausted && __popcll(idle) >= (unsigned)kFetchMin) {
-			int leader = __ffsll((unsigned long long)idle) - 1;
-			unsigned base = 0;
-			if (lane == leader)
-				base = atomicAdd(cursor, (unsigned)__popcll(idle));
-			base = __shfl(base, leader);
-			if (base + (unsigned)__popcll(idle) >= n)
-				exhausted = true;
-			if (r < 0) {
-				unsigned idx = base + (unsigned)__popcll(idle & lt);
-				if (idx < n) {
-					r = (int)idx;
-					float4 o4 = q.so[idx], d4 = q.sd[idx];
-					const WScene &ws = q.scenes[__float_as_uint(q.sc[idx].w)];
-					nodes = ws.sc.nodes;
-					tris = ws.sc.tris;
-					ray = make_tray(xyz(o4), xyz(d4));
-					wr = make_wray(xyz(d4));
-					tmax = d4.w;
-					grp = mk_group(0u, 1u, 1u << ray.oct);
-					thits = 0;
-					sp = 0;
-					nsh++;
-				}
-			}
-		}
-		if (__ballot(r >= 0) == 0) {
-			if (exhausted || n == 0)
-				break;
-			continue;
-		}
-		if (r < 0)
-			continue;
-		bool done = false, occluded = false;
-		if (thits) {
-			while (thits) {
-				uint32_t k = __builtin_ctz(thits);
-				thits &= thits - 1u;
-				uint32_t ti = tri_base + k;
-				float4 a = tris[3 * ti], b4 = tris[3 * ti + 1], c4 = tris[3 * ti + 2];
-				float tt, uu, vv;
-				if (tri_watertight(wr, ray.o, xyz(a), xyz(b4), xyz(c4), tmax, tt, uu, vv)) {
-					occluded = true;
-					break;
-				}
-			}
-			if (occluded)
-				done = true;
-		} else {
-			uint32_t hits = (uint32_t)(grp >> 40);
-			if (hits) {
-				uint32_t b = __builtin_ctz(hits);
-				hits &= hits - 1u;
-				uint32_t slot = b ^ ray.oct;
-				uint32_t imask = (uint32_t)(grp >> 32) & 0xFFu;
-				uint32_t node = (uint32_t)grp + __builtin_popcount(imask & ((1u << slot) - 1u));
-				if (hits) {
-					uint64_t rest = (grp & 0xFFFFFFFFFFull) | ((uint64_t)hits << 40);
-					if (sp < LDS) {
-						lds[sp * kWBlock + threadIdx.x] = rest;
-						sp++;
-					}
-				}
-				uint32_t wv[20];
-				load_node(nodes, node, wv);
-				uint32_t ih;
-				intersect_children(wv, ray, tmax, ih, thits);
-				grp = mk_group(wv[4], wv[3] >> 24, ih);
-				tri_base = wv[5];
-			} else if (sp > 0) {
-				sp--;
-				grp = lds[sp * kWBlock + threadIdx.x];
-			} else {
-				done = true;
-			}
-		}
-		if (done) {
-			if (!occluded) {
-				float4 o4 = q.so[r], c4 = q.sc[r];
-				const WScene &ws = q.scenes[__float_as_uint(c4.w)];
-				float *fb = ws.out + 3 * (size_t)__float_as_uint(o4.w);
-				atomicAdd(fb, c4.x);
-				atomicAdd(fb + 1, c4.y);
-				atomicAdd(fb + 2, c4.z);
-			}
-			r = -1;
 		}
 	}
 	wave_add_u64(&q.cnt->shadow, nsh);
@@ -502,8 +375,6 @@ void wavefront_render_on(const WaveItem *items, int count, const WaveConfig &cfg
 	int dev = 0;
 	RN_HIP(hipGetDevice(&dev));
 	RN_HIP(hipGetDeviceProperties(&prop, dev));
-	unsigned grid = (unsigned)std::max(1, prop.multiProcessorCount) * 8u;
-	grid = std::min<unsigned>(grid, (cap + kWBlock - 1) / kWBlock);
 	bool deep = max_depth - 1 > kLdsDepth;
 
 	// Stage timing: events bracket each launch; pairs are summed at the end.
@@ -517,23 +388,31 @@ void wavefront_render_on(const WaveItem *items, int count, const WaveConfig &cfg
 		if (begin)
 			stage_of.push_back(stage);
 	};
+	const int bs = cfg.block == 128 ? 128 : 256;
+	unsigned grid = (unsigned)std::max(1, prop.multiProcessorCount) * (bs == 128 ? 16u : 8u);
+	grid = std::min<unsigned>(grid, (cap + bs - 1) / bs);
 	auto iteration = [&](int cur) {
 		mark(0, true);
-		if (deep)
-			k_extend<kLdsDepth, kSpillDepth><<<grid, kWBlock, 0, s>>>(q, cur);
-		else
-			k_extend<kLdsDepth, 0><<<grid, kWBlock, 0, s>>>(q, cur);
+#define RN_LAUNCH(K, ...)                                                                                             \
+	do {                                                                                                               \
+		if (deep && bs == 128)                                                                                         \
+			K<kLdsDepth, kSpillDepth, 128><<<grid, 128, 0, s>>>(__VA_ARGS__);                                          \
+		else if (deep)                                                                                                 \
+			K<kLdsDepth, kSpillDepth, 256><<<grid, 256, 0, s>>>(__VA_ARGS__);                                          \
+		else if (bs == 128)                                                                                            \
+			K<kLdsDepth, 0, 128><<<grid, 128, 0, s>>>(__VA_ARGS__);                                                    \
+		else                                                                                                           \
+			K<kLdsDepth, 0, 256><<<grid, 256, 0, s>>>(__VA_ARGS__);                                                    \
+	} while (0)
+		RN_LAUNCH(k_extend, q, cur);
 		mark(0, false);
 		mark(2, true);
-		if (deep)
-			k_shadow<kLdsDepth, kSpillDepth><<<grid, kWBlock, 0, s>>>(q);
-		else if (kUseDynamicFetch)
-			k_shadow_df<kLdsDepth, 0><<<grid, kWBlock, 0, s>>>(q);
-		else
-			k_shadow<kLdsDepth, 0><<<grid, kWBlock, 0, s>>>(q);
+		RN_LAUNCH(k_shadow, q);
 		mark(2, false);
+#undef RN_LAUNCH
 		k_advance<<<1, 1, 0, s>>>(q, 0);
 	};
+
 
 	k_init<<<1, 1, 0, s>>>(q.ctl, total);
 	k_advance<<<1, 1, 0, s>>>(q, 1);

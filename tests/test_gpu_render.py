@@ -121,3 +121,20 @@ def test_single_triangle_scene(rn):
     cam = rn.Camera.make((0, 0, 0), (0, 0, 5), (0, 1, 0), 10.0)
     img = sc.render_hdr(cam, rn.RenderParams.make(8, 8, 4, 3, seed=0, ray_eps=1e-3))
     assert np.allclose(img, 2.0)  # every camera ray sees the emitter
+
+
+def test_autotuner_picks_and_caches(rn, tmp_path, monkeypatch):
+    from rednoise_amd import autotune
+    monkeypatch.setenv("RN_AUTOTUNE_CACHE", str(tmp_path / "tune.json"))
+    items = _items(rn)
+    cands = [dict(queue_paths=2048, bvh_width=8, block_size=256, iters_per_graph=4),
+             dict(queue_paths=8192, bvh_width=4, block_size=128, iters_per_graph=8)]
+    opts, rep = autotune.autotune_batch(items, frac=0.5, candidates=cands)
+    assert not rep["cached"] and len(rep["candidates"]) == 2
+    assert opts.use_graph == 1 and opts.queue_paths in (2048, 8192)
+    opts2, rep2 = autotune.autotune_batch(items, frac=0.5, candidates=cands)
+    assert rep2["cached"] and opts2.queue_paths == opts.queue_paths and opts2.bvh_width == opts.bvh_width
+    imgs = rn.render_batch(items, opts2)
+    for (sc, cam, p), img in zip(items, imgs):
+        p.bvh_width = 8
+        assert np.allclose(img, sc.render_hdr(cam, p), rtol=1e-4, atol=1e-5)
