@@ -592,8 +592,9 @@ static int render_common(const rn_scene *scene, const rn_camera *camera, const r
 		launch_mega(ds, cam, p, out, cnt, (params->flags & RN_FLAG_COUNT_TRAVERSAL) != 0, s);
 	}
 	RN_HIP(hipEventRecord(c.ev1, s));
-	DevCounters hc;
-	download<DevCounters>(&hc, cnt, 1, s);
+	DevCounters hcs;
+	download<DevCounters>(&hcs, cnt, 1, s);
+	const CounterSet hc = hcs.total();
 	if (h_out)
 		download<float>(h_out, out, npx * 3, s);
 	RN_HIP(hipStreamSynchronize(s));
@@ -754,8 +755,9 @@ int rn_render_batch(const rn_batch_item *items, int count, const rn_batch_option
 	RN_HIP(hipEventRecord(c.ev0, c.stream));
 	wavefront_render(w.data(), count, cfg, cnt, c.stream, ws);
 	RN_HIP(hipEventRecord(c.ev1, c.stream));
-	DevCounters hc;
-	download<DevCounters>(&hc, cnt, 1, c.stream);
+	DevCounters hcs;
+	download<DevCounters>(&hcs, cnt, 1, c.stream);
+	const CounterSet hc = hcs.total();
 	for (int i = 0; i < count; i++)
 		download<float>(items[i].hdr_rgb, out + offs[(size_t)i], offs[(size_t)i + 1] - offs[(size_t)i], c.stream);
 	RN_HIP(hipStreamSynchronize(c.stream));

@@ -141,14 +141,15 @@ __global__ __launch_bounds__(kMBlock) void k_mega(DevScene sc, DevCamera cam, De
 		out[o + 1] = sum.y * inv;
 		out[o + 2] = sum.z * inv;
 	}
-	wave_add_u64(&cnt->closest, ncl);
-	wave_add_u64(&cnt->shadow, nsh);
-	wave_add_u64(&cnt->paths, valid ? (unsigned long long)p.spp : 0ull);
+	CounterSet *cs_ = &cnt->shard[blockIdx.x % kCounterShards];
+	wave_add_u64(&cs_->closest, ncl);
+	wave_add_u64(&cs_->shadow, nsh);
+	wave_add_u64(&cs_->paths, valid ? (unsigned long long)p.spp : 0ull);
 	if (COUNT) {
-		wave_add_u64(&cnt->nodes_closest, cc[0]);
-		wave_add_u64(&cnt->tris_closest, cc[1]);
-		wave_add_u64(&cnt->nodes_shadow, cs[0]);
-		wave_add_u64(&cnt->tris_shadow, cs[1]);
+		wave_add_u64(&cs_->nodes_closest, cc[0]);
+		wave_add_u64(&cs_->tris_closest, cc[1]);
+		wave_add_u64(&cs_->nodes_shadow, cs[0]);
+		wave_add_u64(&cs_->tris_shadow, cs[1]);
 	}
 }
 

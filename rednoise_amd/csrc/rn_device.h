@@ -244,132 +244,158 @@ RN_DEV uint64_t mk_group(uint32_t base, uint32_t imask, uint32_t hits) {
 	return (uint64_t)base | ((uint64_t)imask << 32) | ((uint64_t)hits << 40);
 }
 
-// Triangle tests are postponed while fewer than 1/kPostponeDen of the
-// traversing lanes of a wave want them (0 = never). Measured on MI355X at
-// 1M triangles, postponing at 1/3, 1/5 or 1/8 was 4-8% slower than testing
-// triangles at once, so it is off by default.
-#ifndef RN_POSTPONE_DEN
-#define RN_POSTPONE_DEN 0
-#endif
-constexpr uint32_t kPostponeDen = RN_POSTPONE_DEN;
-
 struct Hit {
 	int prim;      // original triangle id, -1 = miss (any-hit: 1 = occluded)
 	uint32_t leaf; // leaf-ordered triangle index of the hit
 	float t, u, v;
 };
 
-// Wide-BVH traversal, one ray per lane. The short stack lives in LDS: entry k
-// of lane `lane` is lds[k * ldsStride + lane]; entries beyond LDS_DEPTH spill
-// to a lane-private array of SPILL entries. The stack never holds more than
-// one entry per tree level below the root, so the host picks an instance with
-// LDS_DEPTH + SPILL >= depth - 1; a deeper push is dropped rather than
-// written out of bounds. QUERY 0 = closest hit, 1 = any hit (early exit).
-// TRI 0 = watertight, 1 = reference (compat) test. Triangles equal to
-// `ignore` (original id) are skipped.
-// With COUNT, cnt[0] += nodes visited and cnt[1] += triangles tested.
+// Scene buffers are always global memory; loading through address space 1
+// gives global_load (vmcnt only) instead of flat_load, which would also hold
+// lgkmcnt and serialise against the LDS stack.
+typedef unsigned int nu4 __attribute__((ext_vector_type(4)));
+typedef float nf4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) const nu4 g_nu4;
+typedef __attribute__((address_space(1))) const nf4 g_nf4;
+
+RN_DEV float4 gload_f4(const float4 *base, uint32_t i) {
+	nf4 v = ((g_nf4 *)base)[i];
+	return make_float4(v.x, v.y, v.z, v.w);
+}
+
+RN_DEV void load_node_g(const Bvh8Node *nodes, uint32_t i, uint32_t w[20]) {
+	g_nu4 *p = (g_nu4 *)(nodes + i);
+#pragma unroll
+	for (int k = 0; k < 5; k++) {
+		nu4 q = p[k];
+		w[4 * k] = q.x;
+		w[4 * k + 1] = q.y;
+		w[4 * k + 2] = q.z;
+		w[4 * k + 3] = q.w;
+	}
+}
+
+// Resumable wide-BVH traversal of one ray, one ray per lane. trav_step runs
+// one iteration (visit at most one node, test the triangles of the leaves it
+// hit, pop the stack when the current group is exhausted) and returns true
+// when the query is answered, so a lane can be refilled with a new ray while
+// the other lanes of its wave are still traversing.
+//
+// The short stack lives in LDS: entry k of lane `lane` is lds[k * ldsStride +
+// lane]; entries beyond LDS_DEPTH spill to a lane-private array of SPILL
+// entries. The stack never holds more than one entry per tree level below the
+// root, so the host picks an instance with LDS_DEPTH + SPILL >= depth - 1; a
+// deeper push is dropped rather than written out of bounds. QUERY 0 = closest
+// hit, 1 = any hit (early exit). TRI 0 = watertight, 1 = reference (compat)
+// test. Triangles equal to `ignore` (original id) are skipped. With COUNT,
+// cnt[0] += nodes visited and cnt[1] += triangles tested.
+template <int SPILL>
+struct Trav {
+	TRay r;
+	WRay wr;
+	uint64_t grp;
+	uint32_t tri_base, thits;
+	int sp;
+	Hit h;
+	uint64_t spill[SPILL > 0 ? SPILL : 1];
+};
+
+template <int TRI, int SPILL>
+RN_DEV void trav_init(Trav<SPILL> &s, float3 o, float3 d, float tmax) {
+	s.r = make_tray(o, d);
+	if (TRI == 0)
+		s.wr = make_wray(d);
+	s.h.prim = -1;
+	s.h.leaf = 0;
+	s.h.t = tmax;
+	s.h.u = 0.0f;
+	s.h.v = 0.0f;
+	s.sp = 0;
+	s.grp = mk_group(0u, 1u, 1u << (0u ^ s.r.oct));
+	s.tri_base = 0;
+	s.thits = 0;
+}
+
+template <int QUERY, int TRI, int LDS_DEPTH, int SPILL, bool COUNT = false>
+RN_DEV bool trav_step(Trav<SPILL> &s, const Bvh8Node *__restrict__ nodes, const float4 *__restrict__ tris, int ignore,
+                      uint64_t *lds, int ldsStride, int lane, uint32_t *cnt = nullptr) {
+	uint32_t hits = (uint32_t)(s.grp >> 40);
+	if (hits) {
+		uint32_t b = __builtin_ctz(hits);
+		hits &= hits - 1u;
+		uint32_t slot = b ^ s.r.oct;
+		uint32_t imask = (uint32_t)(s.grp >> 32) & 0xFFu;
+		uint32_t node = (uint32_t)s.grp + __builtin_popcount(imask & ((1u << slot) - 1u));
+		if (hits) {
+			uint64_t rest = (s.grp & 0xFFFFFFFFFFull) | ((uint64_t)hits << 40);
+			if (s.sp < LDS_DEPTH) {
+				lds[s.sp * ldsStride + lane] = rest;
+				s.sp++;
+			} else if (SPILL > 0 && s.sp < LDS_DEPTH + SPILL) {
+				s.spill[s.sp - LDS_DEPTH] = rest;
+				s.sp++;
+			}
+		}
+		uint32_t w[20];
+		load_node_g(nodes, node, w);
+		if (COUNT)
+			cnt[0]++;
+		uint32_t ih;
+		intersect_children(w, s.r, s.h.t, ih, s.thits);
+		s.grp = mk_group(w[4], w[3] >> 24, ih);
+		s.tri_base = w[5];
+	}
+	if (s.thits) {
+		uint32_t m = s.thits;
+		s.thits = 0;
+		while (m) {
+			uint32_t k = __builtin_ctz(m);
+			m &= m - 1u;
+			uint32_t ti = s.tri_base + k;
+			float4 a = gload_f4(tris, 3 * ti), b4 = gload_f4(tris, 3 * ti + 1), c4 = gload_f4(tris, 3 * ti + 2);
+			int prim = __float_as_int(a.w);
+			if (prim == ignore)
+				continue;
+			if (COUNT)
+				cnt[1]++;
+			float tt, uu, vv;
+			bool ok;
+			if (TRI == 0)
+				ok = tri_watertight(s.wr, s.r.o, xyz(a), xyz(b4), xyz(c4), s.h.t, tt, uu, vv);
+			else
+				ok = tri_compat(s.r.o, s.r.d, xyz(a), xyz(b4), xyz(c4), tt, uu, vv) && tt < s.h.t;
+			if (ok) {
+				if (QUERY == 1) {
+					s.h.prim = 1;
+					return true;
+				}
+				s.h.prim = prim;
+				s.h.leaf = ti;
+				s.h.t = tt;
+				s.h.u = uu;
+				s.h.v = vv;
+			}
+		}
+	}
+	if ((s.grp >> 40) == 0) {
+		if (s.sp == 0) {
+			if (QUERY == 1)
+				s.h.prim = 0;
+			return true;
+		}
+		s.sp--;
+		s.grp = (SPILL == 0 || s.sp < LDS_DEPTH) ? lds[s.sp * ldsStride + lane] : s.spill[s.sp - LDS_DEPTH];
+	}
+	return false;
+}
+
+// Whole-query form of the traversal above.
 template <int QUERY, int TRI, int LDS_DEPTH, int SPILL, bool COUNT = false>
 RN_DEV Hit traverse(const Bvh8Node *__restrict__ nodes, const float4 *__restrict__ tris, float3 o, float3 d,
                     float tmax, int ignore, uint64_t *lds, int ldsStride, int lane, uint32_t *cnt = nullptr) {
-	TRay r = make_tray(o, d);
-	WRay wr;
-	if (TRI == 0)
-		wr = make_wray(d);
-	Hit h;
-	h.prim = -1;
-	h.leaf = 0;
-	h.t = tmax;
-	h.u = 0.0f;
-	h.v = 0.0f;
-	uint64_t spill[SPILL > 0 ? SPILL : 1];
-	int sp = 0;
-	uint64_t grp = mk_group(0u, 1u, 1u << (0u ^ r.oct));
-	uint32_t tri_base = 0, thits = 0;
-	// One postponed triangle group per lane (register-resident, so the stack
-	// bound is unchanged): when fewer than 1/kPostponeDen of the wave's
-	// traversing lanes want triangles, a lane that still has nodes to visit
-	// keeps its new leaf for later instead of idling the others.
-	uint32_t dbase = 0, dhits = 0;
-	while (true) {
-		uint32_t hits = (uint32_t)(grp >> 40);
-		if (hits) {
-			uint32_t b = __builtin_ctz(hits);
-			hits &= hits - 1u;
-			uint32_t slot = b ^ r.oct;
-			uint32_t imask = (uint32_t)(grp >> 32) & 0xFFu;
-			uint32_t node = (uint32_t)grp + __builtin_popcount(imask & ((1u << slot) - 1u));
-			if (hits) {
-				uint64_t rest = (grp & 0xFFFFFFFFFFull) | ((uint64_t)hits << 40);
-				if (sp < LDS_DEPTH) {
-					lds[sp * ldsStride + lane] = rest;
-					sp++;
-				} else if (SPILL > 0 && sp < LDS_DEPTH + SPILL) {
-					spill[sp - LDS_DEPTH] = rest;
-					sp++;
-				}
-			}
-			uint32_t w[20];
-			load_node(nodes, node, w);
-			if (COUNT)
-				cnt[0]++;
-			uint32_t ih;
-			intersect_children(w, r, h.t, ih, thits);
-			grp = mk_group(w[4], w[3] >> 24, ih);
-			tri_base = w[5];
-		}
-		const uint64_t active = __ballot(1);
-		const uint64_t want = __ballot((thits | dhits) != 0u);
-		bool defer = kPostponeDen > 0 && thits != 0u && dhits == 0u && (grp >> 40) != 0 &&
-		             (uint32_t)__popcll(want) * kPostponeDen < (uint32_t)__popcll(active);
-		if (defer) {
-			dbase = tri_base;
-			dhits = thits;
-			thits = 0;
-		} else if (thits | dhits) {
-			// run the current group, then the postponed one
-			for (int g = 0; g < 2; g++) {
-				uint32_t base = g == 0 ? tri_base : dbase;
-				uint32_t m = g == 0 ? thits : dhits;
-				while (m) {
-					uint32_t k = __builtin_ctz(m);
-					m &= m - 1u;
-					uint32_t ti = base + k;
-					float4 a = tris[3 * ti], b4 = tris[3 * ti + 1], c4 = tris[3 * ti + 2];
-					int prim = __float_as_int(a.w);
-					if (prim == ignore)
-						continue;
-					if (COUNT)
-						cnt[1]++;
-					float tt, uu, vv;
-					bool ok;
-					if (TRI == 0)
-						ok = tri_watertight(wr, o, xyz(a), xyz(b4), xyz(c4), h.t, tt, uu, vv);
-					else
-						ok = tri_compat(o, d, xyz(a), xyz(b4), xyz(c4), tt, uu, vv) && tt < h.t;
-					if (ok) {
-						if (QUERY == 1) {
-							h.prim = 1;
-							return h;
-						}
-						h.prim = prim;
-						h.leaf = ti;
-						h.t = tt;
-						h.u = uu;
-						h.v = vv;
-					}
-				}
-			}
-			thits = 0;
-			dhits = 0;
-		}
-		if ((grp >> 40) == 0) {
-			if (sp == 0)
-				break;
-			sp--;
-			grp = (SPILL == 0 || sp < LDS_DEPTH) ? lds[sp * ldsStride + lane] : spill[sp - LDS_DEPTH];
-		}
+	Trav<SPILL> s;
+	trav_init<TRI>(s, o, d, tmax);
+	while (!trav_step<QUERY, TRI, LDS_DEPTH, SPILL, COUNT>(s, nodes, tris, ignore, lds, ldsStride, lane, cnt)) {
 	}
-	if (QUERY == 1)
-		h.prim = 0;
-	return h;
+	return s.h;
 }

@@ -14,10 +14,30 @@ constexpr int kLdsDepth = 12;
 constexpr int kSpillDepth = 52;
 constexpr int kMaxBvhDepth = kLdsDepth + kSpillDepth + 1;
 
-// Counters written by the integrators (device memory, 64-bit).
-struct DevCounters {
+// Counters written by the integrators (device memory, 64-bit). Block b adds
+// into shard b % kCounterShards and the host sums the shards: atomics on one
+// address serialise (about 11 ns each on MI355X), so a single counter bumped
+// once per wave would cost tens of microseconds per launch.
+constexpr int kCounterShards = 64;
+struct CounterSet {
 	unsigned long long closest, shadow, paths, pad;
 	unsigned long long nodes_closest, tris_closest, nodes_shadow, tris_shadow;
+};
+struct DevCounters {
+	CounterSet shard[kCounterShards];
+	CounterSet total() const {
+		CounterSet t{};
+		for (const CounterSet &c : shard) {
+			t.closest += c.closest;
+			t.shadow += c.shadow;
+			t.paths += c.paths;
+			t.nodes_closest += c.nodes_closest;
+			t.tris_closest += c.tris_closest;
+			t.nodes_shadow += c.nodes_shadow;
+			t.tris_shadow += c.tris_shadow;
+		}
+		return t;
+	}
 };
 
 void launch_trace_rays(const DevScene &sc, int n, const float *o, const float *d, const float *tmax,

@@ -2,16 +2,21 @@
 // "Megakernels Considered Harmful", HPG 2013), for one scene or a batch of
 // scenes of uneven size that share one path queue.
 //
-// Per iteration three persistent, grid-stride kernels run:
-//   extend  - appends freshly generated camera paths to the queue, traces the
-//             closest hit of every queued path (SoA state), then shades it:
-//             emission, NEE light sample, bounce and Russian roulette; shadow
-//             rays and continuing paths are pushed with one wave64 ballot and
-//             one atomic per wave (compaction keeps the next queue dense);
-//   shadow  - any-hit queries; unoccluded contributions are added to the
-//             frame buffer with float atomics;
-//   advance - one lane: rotates the queues and plans the next generation so
-//             the queue stays full until the work runs out.
+// Per iteration five kernels run:
+//   generate - appends freshly generated camera paths to the queue;
+//   extend   - closest hit of every queued path (SoA state);
+//   shade    - emission, NEE light sample, bounce and Russian roulette;
+//              shadow rays and continuing paths are pushed with one wave64
+//              ballot and one atomic per wave (compaction keeps the next
+//              queue dense);
+//   shadow   - any-hit queries; unoccluded contributions are added to the
+//              frame buffer with float atomics;
+//   advance  - one lane: rotates the queues and plans the next generation so
+//              the queue stays full until the work runs out.
+// The two trace kernels are persistent: a wave takes chunks of queue entries
+// from an atomic counter and gives a lane whose ray has finished the next
+// ray of the chunk as soon as enough lanes are idle, so a wave no longer
+// runs as long as its slowest ray once per 64 rays.
 // Queue sizes live in device memory, so the iteration can be captured once in
 // a hipGraph and replayed; the host only polls a done flag between replays.
 #include "rn_pathp.h"
@@ -37,12 +42,14 @@ struct WScene {
 
 struct WCtl {
 	unsigned cur_count, gen_count, next_count, sh_count;
+	unsigned trace_next, shadow_next; // chunk counters of the trace kernels
 	unsigned long long gen_base, work_next, work_total;
 	unsigned done, iter;
 };
 
 struct WQ {
 	float4 *o[2], *d[2], *b[2];
+	uint4 *hit; // per queue entry: prim (-1 = miss), leaf index, t
 	float4 *so, *sd, *sc;
 	WCtl *ctl;
 	const WScene *scenes;
@@ -58,6 +65,7 @@ __global__ void k_fill(uint32_t *p, size_t n) {
 
 __global__ void k_init(WCtl *ctl, unsigned long long total) {
 	ctl->cur_count = ctl->gen_count = ctl->next_count = ctl->sh_count = 0;
+	ctl->trace_next = ctl->shadow_next = 0;
 	ctl->gen_base = 0;
 	ctl->work_next = 0;
 	ctl->work_total = total;
@@ -72,6 +80,8 @@ __global__ void k_advance(WQ q, int first) {
 		c->cur_count = c->next_count;
 	c->next_count = 0;
 	c->sh_count = 0;
+	c->trace_next = 0;
+	c->shadow_next = 0;
 	unsigned long long remaining = c->work_total - c->work_next;
 	unsigned long long room = (unsigned long long)(q.capacity - c->cur_count);
 	unsigned gen = (unsigned)(remaining < room ? remaining : room);
@@ -97,96 +107,196 @@ RN_DEV int find_scene(const WScene *sc, int n, unsigned long long g) {
 // meta word: scene index (low 16 bits) | depth << 16; depth 255 = dead slot.
 constexpr uint32_t kDead = 255u;
 
-// Push with one atomic per wave: returns this lane's slot (valid when push).
-RN_DEV unsigned wave_push(unsigned *counter, bool push) {
-	uint64_t m = __ballot(push);
-	if (m == 0)
-		return 0;
-	int lane = threadIdx.x & 63;
-	int leader = __ffsll((unsigned long long)m) - 1;
-	unsigned base = 0;
-	if (lane == leader)
-		base = atomicAdd(counter, (unsigned)__popcll(m));
-	base = __shfl(base, leader);
-	uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-	return base + (unsigned)__popcll(m & lt);
-}
-
-// Extend + shade: generate the new camera paths appended to the queue, trace
-// every queued path's closest hit, then (all lanes of the wave together)
-// add emission, draw the NEE light sample, the bounce and the roulette, and
-// push the shadow ray and the continuing path.
-template <int LDS, int SPILL, int BS>
-__global__ __launch_bounds__(BS) void k_extend(WQ q, int cur) {
-	__shared__ uint64_t stk[LDS * BS];
+// New camera paths for queue entries [cur_count, cur_count + gen_count).
+__global__ __launch_bounds__(256) void k_generate(WQ q, int cur) {
 	WCtl *c = q.ctl;
 	const unsigned ncur = c->cur_count, n = ncur + c->gen_count;
 	const unsigned long long gbase = c->gen_base;
 	float4 *O = q.o[cur], *D = q.d[cur], *B = q.b[cur];
-	float4 *NO = q.o[cur ^ 1], *ND = q.d[cur ^ 1], *NB = q.b[cur ^ 1];
-	unsigned long long ncl = 0, npaths = 0;
-	for (unsigned base = blockIdx.x * BS; base < n; base += gridDim.x * BS) {
+	unsigned long long npaths = 0;
+	for (unsigned base = ncur + blockIdx.x * 256u; base < n; base += gridDim.x * 256u) {
 		const unsigned i = base + threadIdx.x;
-		float4 o4 = make_float4(0.f, 0.f, 0.f, 0.f), d4 = o4, b4 = make_float4(0.f, 0.f, 0.f, 0.f);
-		uint32_t meta = kDead << 16;
-		if (i < n) {
-			if (i >= ncur) { // new camera path
-				unsigned long long g = gbase + (i - ncur);
-				int si = find_scene(q.scenes, q.nscenes, g);
-				const WScene &ws = q.scenes[si];
-				unsigned long long local = g - ws.first;
-				uint32_t smp = (uint32_t)(local / (unsigned long long)ws.npix_pad);
-				uint32_t pp = (uint32_t)(local % (unsigned long long)ws.npix_pad);
-				uint32_t tile = pp >> 6, t = pp & 63u;
-				int x = (int)(tile % (uint32_t)ws.tiles_x) * kPixTile + (int)(t & 7u);
-				int y = (int)(tile / (uint32_t)ws.tiles_x) * kPixTile + (int)(t >> 3);
-				uint32_t pix = (uint32_t)(y * ws.p.width + x);
-				meta = (uint32_t)si | (kDead << 16);
-				if (x < ws.p.width && y < ws.p.height) {
-					u32x4 r = philox4x32_10(pix, smp, 0u, 0u, ws.p.seed, ws.p.scene_id);
-					float3 d = camera_dir(ws.cam, x, y, ws.p.width, ws.p.height, u01(r.x), u01(r.y));
-					o4 = make_float4(ws.cam.eye[0], ws.cam.eye[1], ws.cam.eye[2], __uint_as_float(pix));
-					d4 = make_float4(d.x, d.y, d.z, __uint_as_float(smp));
-					b4 = make_float4(1.f, 1.f, 1.f, 0.f);
-					meta = (uint32_t)si;
-					npaths++;
+		if (i >= n)
+			continue;
+		unsigned long long g = gbase + (i - ncur);
+		int si = find_scene(q.scenes, q.nscenes, g);
+		const WScene &ws = q.scenes[si];
+		unsigned long long local = g - ws.first;
+		uint32_t smp = (uint32_t)(local / (unsigned long long)ws.npix_pad);
+		uint32_t pp = (uint32_t)(local % (unsigned long long)ws.npix_pad);
+		uint32_t tile = pp >> 6, t = pp & 63u;
+		int x = (int)(tile % (uint32_t)ws.tiles_x) * kPixTile + (int)(t & 7u);
+		int y = (int)(tile / (uint32_t)ws.tiles_x) * kPixTile + (int)(t >> 3);
+		float4 o4 = make_float4(0.f, 0.f, 0.f, 0.f), d4 = o4;
+		uint32_t meta = (uint32_t)si | (kDead << 16);
+		if (x < ws.p.width && y < ws.p.height) {
+			uint32_t pix = (uint32_t)(y * ws.p.width + x);
+			u32x4 r = philox4x32_10(pix, smp, 0u, 0u, ws.p.seed, ws.p.scene_id);
+			float3 d = camera_dir(ws.cam, x, y, ws.p.width, ws.p.height, u01(r.x), u01(r.y));
+			o4 = make_float4(ws.cam.eye[0], ws.cam.eye[1], ws.cam.eye[2], __uint_as_float(pix));
+			d4 = make_float4(d.x, d.y, d.z, __uint_as_float(smp));
+			meta = (uint32_t)si;
+			npaths++;
+		}
+		O[i] = o4;
+		D[i] = d4;
+		B[i] = make_float4(1.f, 1.f, 1.f, __uint_as_float(meta));
+	}
+	wave_add_u64(&q.cnt->shard[blockIdx.x % kCounterShards].paths, npaths);
+}
+
+// Queue entries a wave takes from the chunk counter at a time, and the number
+// of idle lanes that triggers a refill.
+#ifndef RN_TRACE_CHUNK
+#define RN_TRACE_CHUNK 128
+#endif
+#ifndef RN_REFILL_MIN
+#define RN_REFILL_MIN 16
+#endif
+constexpr unsigned kChunk = RN_TRACE_CHUNK;
+constexpr int kRefillMin = RN_REFILL_MIN;
+static_assert(kRefillMin >= 1 && kRefillMin <= 64, "refill threshold is a lane count");
+
+// Persistent trace over a queue. QUERY 0: closest hit of path-queue entries
+// [0, cur_count + gen_count), written to q.hit; dead entries are skipped.
+// QUERY 1: any hit of shadow-queue entries [0, sh_count); unoccluded
+// contributions go to the frame buffer.
+//
+// Every branch that decides on refills or on leaving the loop depends only on
+// ballots and on values read from lane 0, so it is uniform over the wave and
+// all 64 lanes reach each __ballot/__shfl together. The loop ends once the
+// chunk counter has passed the queue length and every lane is idle.
+template <int QUERY, int LDS, int SPILL, int BS>
+RN_DEV void trace_queue(const WQ &q, int cur, uint64_t *stk) {
+	WCtl *c = q.ctl;
+	const unsigned n = QUERY == 0 ? c->cur_count + c->gen_count : c->sh_count;
+	unsigned *ctr = QUERY == 0 ? &c->trace_next : &c->shadow_next;
+	const float4 *O = QUERY == 0 ? q.o[cur] : q.so;
+	const float4 *D = QUERY == 0 ? q.d[cur] : q.sd;
+	const float4 *M = QUERY == 0 ? q.b[cur] : q.sc; // .w: meta (closest) or scene index (shadow)
+	const int lane = threadIdx.x & 63;
+	const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+	unsigned next = 0, end = 0;
+	bool more = true, valid = false;
+	unsigned idx = 0;
+	const Bvh8Node *nodes = nullptr;
+	const float4 *tris = nullptr;
+	Trav<SPILL> t;
+	unsigned long long nrays = 0;
+	while (true) {
+		const uint64_t idle = __ballot(!valid);
+		if (more && __popcll(idle) >= kRefillMin) {
+			if (next >= end) {
+				unsigned cb = 0;
+				if (lane == 0)
+					cb = atomicAdd(ctr, kChunk);
+				cb = __shfl(cb, 0);
+				if (cb >= n) {
+					more = false;
+				} else {
+					next = cb;
+					end = min(cb + kChunk, n);
 				}
-			} else {
-				o4 = O[i];
-				d4 = D[i];
-				b4 = B[i];
-				meta = __float_as_uint(b4.w);
+			}
+			if (next < end) {
+				const unsigned my = next + (unsigned)__popcll(idle & lt);
+				if (!valid && my < end) {
+					const uint32_t meta = __float_as_uint(M[my].w);
+					if (QUERY == 1 || (meta >> 16) != kDead) {
+						const DevScene &sc = q.scenes[meta & 0xFFFFu].sc;
+						nodes = sc.nodes;
+						tris = sc.tris;
+						const float4 o4 = O[my], d4 = D[my];
+						trav_init<0>(t, xyz(o4), xyz(d4), QUERY == 0 ? INFINITY : d4.w);
+						valid = true;
+						idx = my;
+						nrays++;
+					}
+				}
+				next = min(next + (unsigned)__popcll(idle), end);
 			}
 		}
-		if (i < n && i >= ncur) { // publish the new path so it can be reloaded after traversal
-			O[i] = o4;
-			D[i] = d4;
-			B[i] = make_float4(b4.x, b4.y, b4.z, __uint_as_float(meta));
+		if (__ballot(valid) == 0) {
+			if (!more)
+				break;
+			continue;
+		}
+		if (valid && trav_step<QUERY, 0, LDS, SPILL>(t, nodes, tris, -1, stk, BS, threadIdx.x)) {
+			valid = false;
+			if (QUERY == 0) {
+				q.hit[idx] = make_uint4((uint32_t)t.h.prim, t.h.leaf, __float_as_uint(t.h.t), 0u);
+			} else if (t.h.prim == 0) {
+				const float4 c4 = q.sc[idx];
+				const WScene &ws = q.scenes[__float_as_uint(c4.w)];
+				float *fb = ws.out + 3 * (size_t)__float_as_uint(q.so[idx].w);
+				atomicAdd(fb, c4.x);
+				atomicAdd(fb + 1, c4.y);
+				atomicAdd(fb + 2, c4.z);
+			}
+		}
+	}
+	CounterSet *cs = &q.cnt->shard[blockIdx.x % kCounterShards];
+	wave_add_u64(QUERY == 0 ? &cs->closest : &cs->shadow, nrays);
+}
+
+template <int LDS, int SPILL, int BS>
+__global__ __launch_bounds__(BS) void k_extend(WQ q, int cur) {
+	__shared__ uint64_t stk[LDS * BS];
+	trace_queue<0, LDS, SPILL, BS>(q, cur, stk);
+}
+
+template <int LDS, int SPILL, int BS>
+__global__ __launch_bounds__(BS) void k_shadow(WQ q) {
+	__shared__ uint64_t stk[LDS * BS];
+	trace_queue<1, LDS, SPILL, BS>(q, 0, stk);
+}
+
+// Shade every queued path at its closest hit: emission (camera rays only),
+// the NEE light sample, the bounce and the roulette; push the shadow ray and
+// the continuing path. Pushes are counted per block: one atomic per queue per
+// 1024 entries (same-address atomics serialise at about 11 ns each on
+// MI355X, so one per wave would bound this kernel).
+constexpr int kShadeBlock = 1024;
+constexpr int kShadeWaves = kShadeBlock / 64;
+
+__global__ __launch_bounds__(kShadeBlock) void k_shade(WQ q, int cur) {
+	__shared__ unsigned wcnt[2][kShadeWaves];
+	__shared__ unsigned bbase[2];
+	WCtl *c = q.ctl;
+	const unsigned n = c->cur_count + c->gen_count;
+	const float4 *O = q.o[cur], *D = q.d[cur], *B = q.b[cur];
+	float4 *NO = q.o[cur ^ 1], *ND = q.d[cur ^ 1], *NB = q.b[cur ^ 1];
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+	for (unsigned base = blockIdx.x * (unsigned)kShadeBlock; base < n; base += gridDim.x * (unsigned)kShadeBlock) {
+		const unsigned i = base + threadIdx.x;
+		uint32_t meta = kDead << 16;
+		float4 b4 = make_float4(0.f, 0.f, 0.f, 0.f);
+		if (i < n) {
+			b4 = B[i];
+			meta = __float_as_uint(b4.w);
 		}
 		const uint32_t depth = meta >> 16;
 		bool push_sh = false, push_next = false;
 		float3 so = f3(0.f, 0.f, 0.f), wo = so, beta = so, contrib = so, shw = so;
 		float shmax = 0.f;
+		float4 o4 = make_float4(0.f, 0.f, 0.f, 0.f), d4 = o4;
 		if (depth != kDead) {
-			const DevScene &sc0 = q.scenes[meta & 0xFFFFu].sc;
-			Hit h = traverse<0, 0, LDS, SPILL>(sc0.nodes, sc0.tris, xyz(o4), xyz(d4), INFINITY, -1, stk, BS,
-			                                   threadIdx.x);
-			ncl++;
-			if (h.prim >= 0) {
-				// reload the state rather than keeping it live across traversal
+			const uint4 hr = q.hit[i];
+			const int prim = (int)hr.x;
+			if (prim >= 0) {
 				o4 = O[i];
 				d4 = D[i];
-				b4 = B[i];
 				beta = xyz(b4);
 				const WScene &ws = q.scenes[meta & 0xFFFFu];
 				const DevScene &sc = ws.sc;
 				const DevParams &p = ws.p;
 				float3 o = xyz(o4), d = xyz(d4);
 				uint32_t pix = __float_as_uint(o4.w), smp = __float_as_uint(d4.w);
-				float3 P = o + d * h.t;
-				float3 ng = facing_normal(sc.tris, h.leaf, d);
+				float3 P = o + d * __uint_as_float(hr.z);
+				float3 ng = facing_normal(sc.tris, hr.y, d);
 				if (depth == 0) {
-					float3 le = load3(sc.emission, h.prim);
+					float3 le = load3(sc.emission, prim);
 					if (le.x != 0.f || le.y != 0.f || le.z != 0.f) {
 						float *fb = ws.out + 3 * (size_t)pix;
 						atomicAdd(fb, beta.x * le.x);
@@ -194,7 +304,7 @@ __global__ __launch_bounds__(BS) void k_extend(WQ q, int cur) {
 						atomicAdd(fb + 2, beta.z * le.z);
 					}
 				}
-				float3 a = load3(sc.albedo, h.prim);
+				float3 a = load3(sc.albedo, prim);
 				so = P + ng * p.ray_eps;
 				u32x4 rn = philox4x32_10(pix, smp, depth, 1u, p.seed, p.scene_id);
 				if (sc.nl > 0) {
@@ -222,45 +332,37 @@ __global__ __launch_bounds__(BS) void k_extend(WQ q, int cur) {
 				}
 			}
 		}
-		unsigned ss = wave_push(&c->sh_count, push_sh);
+		// block-wide compaction: wave counts -> one atomic per queue -> offsets
+		const uint64_t msh = __ballot(push_sh), mnx = __ballot(push_next);
+		if (lane == 0) {
+			wcnt[0][wave] = (unsigned)__popcll(msh);
+			wcnt[1][wave] = (unsigned)__popcll(mnx);
+		}
+		__syncthreads();
+		if (threadIdx.x < 2) {
+			unsigned tot = 0;
+			for (int v = 0; v < kShadeWaves; v++)
+				tot += wcnt[threadIdx.x][v];
+			bbase[threadIdx.x] = tot ? atomicAdd(threadIdx.x == 0 ? &c->sh_count : &c->next_count, tot) : 0u;
+		}
+		__syncthreads();
+		unsigned ss = bbase[0] + (unsigned)__popcll(msh & lt), ns = bbase[1] + (unsigned)__popcll(mnx & lt);
+		for (int v = 0; v < wave; v++) {
+			ss += wcnt[0][v];
+			ns += wcnt[1][v];
+		}
+		__syncthreads(); // wcnt and bbase are rewritten by the next iteration
 		if (push_sh) {
 			q.so[ss] = make_float4(so.x, so.y, so.z, o4.w);
 			q.sd[ss] = make_float4(shw.x, shw.y, shw.z, shmax);
 			q.sc[ss] = make_float4(contrib.x, contrib.y, contrib.z, __uint_as_float(meta & 0xFFFFu));
 		}
-		unsigned ns = wave_push(&c->next_count, push_next);
 		if (push_next) {
 			NO[ns] = make_float4(so.x, so.y, so.z, o4.w);
 			ND[ns] = make_float4(wo.x, wo.y, wo.z, d4.w);
 			NB[ns] = make_float4(beta.x, beta.y, beta.z, __uint_as_float((meta & 0xFFFFu) | ((depth + 1) << 16)));
 		}
 	}
-	wave_add_u64(&q.cnt->closest, ncl);
-	wave_add_u64(&q.cnt->paths, npaths);
-}
-
-template <int LDS, int SPILL, int BS>
-__global__ __launch_bounds__(BS) void k_shadow(WQ q) {
-	__shared__ uint64_t stk[LDS * BS];
-	const unsigned n = q.ctl->sh_count;
-	unsigned long long nsh = 0;
-	for (unsigned base = blockIdx.x * BS; base < n; base += gridDim.x * BS) {
-		unsigned i = base + threadIdx.x;
-		if (i >= n)
-			continue;
-		float4 o4 = q.so[i], d4 = q.sd[i], c4 = q.sc[i];
-		const WScene &ws = q.scenes[__float_as_uint(c4.w)];
-		Hit h = traverse<1, 0, LDS, SPILL>(ws.sc.nodes, ws.sc.tris, xyz(o4), xyz(d4), d4.w, -1, stk, BS,
-		                                   threadIdx.x);
-		nsh++;
-		if (h.prim == 0) {
-			float *fb = ws.out + 3 * (size_t)__float_as_uint(o4.w);
-			atomicAdd(fb, c4.x);
-			atomicAdd(fb + 1, c4.y);
-			atomicAdd(fb + 2, c4.z);
-		}
-	}
-	wave_add_u64(&q.cnt->shadow, nsh);
 }
 
 __global__ void k_resolve(float *out, size_t n, float scale) {
@@ -359,6 +461,7 @@ void wavefront_render_on(const WaveItem *items, int count, const WaveConfig &cfg
 	q.o[1] = st4 + 3 * (size_t)cap;
 	q.d[1] = st4 + 4 * (size_t)cap;
 	q.b[1] = st4 + 5 * (size_t)cap;
+	q.hit = static_cast<uint4 *>(ww.hit.reserve((size_t)cap * sizeof(uint4)));
 	float4 *sh4 = static_cast<float4 *>(ww.shadow.reserve((size_t)cap * 3 * sizeof(float4)));
 	q.so = sh4;
 	q.sd = sh4 + cap;
@@ -389,9 +492,19 @@ void wavefront_render_on(const WaveItem *items, int count, const WaveConfig &cfg
 			stage_of.push_back(stage);
 	};
 	const int bs = cfg.block == 128 ? 128 : 256;
+	// Trace kernels: enough resident waves to fill every SIMD; each wave
+	// pulls chunks until the queue is exhausted. Shade and generate: plain
+	// grid-stride launches.
 	unsigned grid = (unsigned)std::max(1, prop.multiProcessorCount) * (bs == 128 ? 16u : 8u);
-	grid = std::min<unsigned>(grid, (cap + bs - 1) / bs);
+	const unsigned chunks = (cap + kChunk - 1) / kChunk, waves_per_block = (unsigned)bs / 64u;
+	grid = std::min<unsigned>(grid, (chunks + waves_per_block - 1) / waves_per_block);
+	const unsigned sgrid = std::min<unsigned>((unsigned)std::max(1, prop.multiProcessorCount) * 8u, (cap + 255) / 256);
+	const unsigned shgrid = std::min<unsigned>((unsigned)std::max(1, prop.multiProcessorCount) * 2u,
+	                                           (cap + kShadeBlock - 1) / kShadeBlock);
 	auto iteration = [&](int cur) {
+		mark(1, true);
+		k_generate<<<sgrid, 256, 0, s>>>(q, cur);
+		mark(1, false);
 		mark(0, true);
 #define RN_LAUNCH(K, ...)                                                                                             \
 	do {                                                                                                               \
@@ -406,6 +519,9 @@ void wavefront_render_on(const WaveItem *items, int count, const WaveConfig &cfg
 	} while (0)
 		RN_LAUNCH(k_extend, q, cur);
 		mark(0, false);
+		mark(1, true);
+		k_shade<<<shgrid, kShadeBlock, 0, s>>>(q, cur);
+		mark(1, false);
 		mark(2, true);
 		RN_LAUNCH(k_shadow, q);
 		mark(2, false);
