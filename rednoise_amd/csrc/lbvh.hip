@@ -474,7 +474,7 @@ __global__ void k_collapse(int k0, int k1, int width, int *task, const float4 *b
 			task[child_base + ic] = q.id;
 			ic++;
 		} else {
-			nd.meta[s] = (uint8_t)((cnt << 5) | off);
+			nd.meta[s] = (uint8_t)((((1u << cnt) - 1u) << 5) | off);
 			for (int r = 0; r < cnt; r++) {
 				uint32_t prim = order[q.first + r];
 				const float *t = verts + 9 * (size_t)prim;

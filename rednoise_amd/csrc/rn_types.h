@@ -11,8 +11,9 @@
 // Compressed Wide BVHs" (HPG 2017).
 //   imask bit s   : slot s is an internal child; internal children are stored
 //                   contiguously from child_base in slot order.
-//   meta[s]       : leaf slot: (triangle count 1..3) << 5 | offset from
-//                   tri_base (0..23); internal or empty slot: 0.
+//   meta[s]       : leaf slot: unary triangle count (1, 3 or 7 for one, two
+//                   or three triangles) << 5 | offset from tri_base (0..21);
+//                   internal or empty slot: 0.
 //   empty slot    : qlo = 255, qhi = 0 on every axis (never hit).
 // Slot s is assigned to the child lying furthest along the diagonal whose
 // component k is +1 when bit k of s is set and -1 otherwise. A ray with sign

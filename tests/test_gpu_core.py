@@ -99,8 +99,9 @@ def check_bvh(verts, nodes_raw, prims, width):
                 clo, chi, cd = visit(c, depth + 1)
                 depth_c = cd
             else:
-                cnt, off = meta >> 5, meta & 31
-                assert 1 <= cnt <= 3
+                unary, off = meta >> 5, meta & 31
+                assert unary in (1, 3, 7), "leaf count is a unary mask"
+                cnt = bin(unary).count("1")
                 idx = prims[int(nd["tri_base"]) + off: int(nd["tri_base"]) + off + cnt]
                 clo = vmin[idx].min(0)
                 chi = vmax[idx].max(0)
