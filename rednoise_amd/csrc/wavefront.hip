@@ -23,6 +23,7 @@
 #include "wavefront.h"
 
 #include <algorithm>
+#include <cstddef>
 #include <vector>
 
 namespace rn {
@@ -40,12 +41,30 @@ struct WScene {
 	int tiles_x, npix_pad;
 };
 
+// Chunk counters of the trace kernels are striped over kStripes addresses,
+// one 128-byte line each, so waves contend on eight counters instead of one.
+constexpr int kStripes = 8;
+constexpr int kStripePad = 32; // unsigneds per counter line
+
 struct WCtl {
-	unsigned cur_count, gen_count, next_count, sh_count;
-	unsigned trace_next, shadow_next; // chunk counters of the trace kernels
+	unsigned cur_count, gen_count;
+	unsigned next_count, sh_count; // one 64-bit word: k_shade bumps both with one atomic
 	unsigned long long gen_base, work_next, work_total;
 	unsigned done, iter;
+	unsigned pad[20];
+	unsigned trace_ctr[kStripes * kStripePad];
+	unsigned shadow_ctr[kStripes * kStripePad];
 };
+static_assert(offsetof(WCtl, next_count) % 8 == 0 && offsetof(WCtl, sh_count) == offsetof(WCtl, next_count) + 4,
+              "next_count and sh_count must form one aligned 64-bit word");
+static_assert(offsetof(WCtl, trace_ctr) % 128 == 0, "stripe counters start on a 128-byte line");
+
+RN_DEV void reset_stripes(WCtl *c) {
+	for (int k = 0; k < kStripes; k++) {
+		c->trace_ctr[k * kStripePad] = 0;
+		c->shadow_ctr[k * kStripePad] = 0;
+	}
+}
 
 struct WQ {
 	float4 *o[2], *d[2], *b[2];
@@ -65,7 +84,7 @@ __global__ void k_fill(uint32_t *p, size_t n) {
 
 __global__ void k_init(WCtl *ctl, unsigned long long total) {
 	ctl->cur_count = ctl->gen_count = ctl->next_count = ctl->sh_count = 0;
-	ctl->trace_next = ctl->shadow_next = 0;
+	reset_stripes(ctl);
 	ctl->gen_base = 0;
 	ctl->work_next = 0;
 	ctl->work_total = total;
@@ -80,8 +99,7 @@ __global__ void k_advance(WQ q, int first) {
 		c->cur_count = c->next_count;
 	c->next_count = 0;
 	c->sh_count = 0;
-	c->trace_next = 0;
-	c->shadow_next = 0;
+	reset_stripes(c);
 	unsigned long long remaining = c->work_total - c->work_next;
 	unsigned long long room = (unsigned long long)(q.capacity - c->cur_count);
 	unsigned gen = (unsigned)(remaining < room ? remaining : room);
@@ -170,7 +188,11 @@ template <int QUERY, int LDS, int SPILL, int BS>
 RN_DEV void trace_queue(const WQ &q, int cur, uint64_t *stk) {
 	WCtl *c = q.ctl;
 	const unsigned n = QUERY == 0 ? c->cur_count + c->gen_count : c->sh_count;
-	unsigned *ctr = QUERY == 0 ? &c->trace_next : &c->shadow_next;
+	unsigned *ctrs = QUERY == 0 ? c->trace_ctr : c->shadow_ctr;
+	// chunk j = stripe + kStripes * k is the k-th chunk of stripe j % kStripes;
+	// a wave starts on its own stripe and moves on when it is exhausted
+	const unsigned nch = (n + kChunk - 1) / kChunk;
+	unsigned stripe = (blockIdx.x * (unsigned)(BS / 64) + (threadIdx.x >> 6)) % (unsigned)kStripes;
 	const float4 *O = QUERY == 0 ? q.o[cur] : q.so;
 	const float4 *D = QUERY == 0 ? q.d[cur] : q.sd;
 	const float4 *M = QUERY == 0 ? q.b[cur] : q.sc; // .w: meta (closest) or scene index (shadow)
@@ -187,11 +209,26 @@ RN_DEV void trace_queue(const WQ &q, int cur, uint64_t *stk) {
 		const uint64_t idle = __ballot(!valid);
 		if (more && __popcll(idle) >= kRefillMin) {
 			if (next >= end) {
-				unsigned cb = 0;
-				if (lane == 0)
-					cb = atomicAdd(ctr, kChunk);
+				unsigned cb = 0xFFFFFFFFu;
+				if (lane == 0) {
+					for (int tries = 0; tries < kStripes; tries++) {
+						unsigned *ctr = ctrs + stripe * kStripePad;
+						// skip a stripe seen exhausted without an atomic
+						unsigned k0 = __hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+						if (stripe + (unsigned)kStripes * k0 < nch) {
+							unsigned k = atomicAdd(ctr, 1u);
+							unsigned j = stripe + (unsigned)kStripes * k;
+							if (j < nch) {
+								cb = j * kChunk;
+								break;
+							}
+						}
+						stripe = (stripe + 1u) % (unsigned)kStripes;
+					}
+				}
 				cb = __shfl(cb, 0);
-				if (cb >= n) {
+				stripe = __shfl(stripe, 0);
+				if (cb == 0xFFFFFFFFu) {
 					more = false;
 				} else {
 					next = cb;
@@ -239,14 +276,25 @@ RN_DEV void trace_queue(const WQ &q, int cur, uint64_t *stk) {
 	wave_add_u64(QUERY == 0 ? &cs->closest : &cs->shadow, nrays);
 }
 
+// Optional occupancy target for the trace kernels (waves per SIMD; 0 = the
+// compiler's choice).
+#ifndef RN_TRACE_WPE
+#define RN_TRACE_WPE 0
+#endif
+#if RN_TRACE_WPE > 0
+#define RN_TRACE_ATTR __attribute__((amdgpu_waves_per_eu(RN_TRACE_WPE, RN_TRACE_WPE)))
+#else
+#define RN_TRACE_ATTR
+#endif
+
 template <int LDS, int SPILL, int BS>
-__global__ __launch_bounds__(BS) void k_extend(WQ q, int cur) {
+__global__ __launch_bounds__(BS) RN_TRACE_ATTR void k_extend(WQ q, int cur) {
 	__shared__ uint64_t stk[LDS * BS];
 	trace_queue<0, LDS, SPILL, BS>(q, cur, stk);
 }
 
 template <int LDS, int SPILL, int BS>
-__global__ __launch_bounds__(BS) void k_shadow(WQ q) {
+__global__ __launch_bounds__(BS) RN_TRACE_ATTR void k_shadow(WQ q) {
 	__shared__ uint64_t stk[LDS * BS];
 	trace_queue<1, LDS, SPILL, BS>(q, 0, stk);
 }
@@ -339,11 +387,18 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(WQ q, int cur) {
 			wcnt[1][wave] = (unsigned)__popcll(mnx);
 		}
 		__syncthreads();
-		if (threadIdx.x < 2) {
-			unsigned tot = 0;
-			for (int v = 0; v < kShadeWaves; v++)
-				tot += wcnt[threadIdx.x][v];
-			bbase[threadIdx.x] = tot ? atomicAdd(threadIdx.x == 0 ? &c->sh_count : &c->next_count, tot) : 0u;
+		if (threadIdx.x == 0) {
+			unsigned tsh = 0, tnx = 0;
+			for (int v = 0; v < kShadeWaves; v++) {
+				tsh += wcnt[0][v];
+				tnx += wcnt[1][v];
+			}
+			unsigned long long old = 0;
+			if (tsh | tnx)
+				old = atomicAdd(reinterpret_cast<unsigned long long *>(&c->next_count),
+				                (unsigned long long)tnx | ((unsigned long long)tsh << 32));
+			bbase[0] = (unsigned)(old >> 32);
+			bbase[1] = (unsigned)old;
 		}
 		__syncthreads();
 		unsigned ss = bbase[0] + (unsigned)__popcll(msh & lt), ns = bbase[1] + (unsigned)__popcll(mnx & lt);
