@@ -81,13 +81,23 @@ RN_DEV TRay make_tray(float3 o, float3 d) {
 	TRay r;
 	r.o = o;
 	r.d = d;
-	const float tiny = 1e-30f;
+	// |1/d| stays below 2^40, so node slab terms scaled by 2^24 (below) stay
+	// finite for any node scale up to 2^60
+	const float tiny = 0x1p-40f;
 	float dx = fabsf(d.x) > tiny ? d.x : copysignf(tiny, d.x);
 	float dy = fabsf(d.y) > tiny ? d.y : copysignf(tiny, d.y);
 	float dz = fabsf(d.z) > tiny ? d.z : copysignf(tiny, d.z);
 	r.inv = f3(1.0f / dx, 1.0f / dy, 1.0f / dz);
 	r.oct = (d.x < 0.0f ? 1u : 0u) | (d.y < 0.0f ? 2u : 0u) | (d.z < 0.0f ? 4u : 0u);
 	return r;
+}
+
+typedef _Float16 h16x2 __attribute__((ext_vector_type(2)));
+
+// Two bytes of w (chosen by the v_perm selector) as two f16 denormals
+// byte * 2^-24, exactly.
+RN_DEV h16x2 h16_pair(uint32_t w, uint32_t sel) {
+	return __builtin_bit_cast(h16x2, __builtin_amdgcn_perm(0u, w, sel));
 }
 
 // Bit b of the result is bit (b ^ oct) of x (x has 8 significant bits).
@@ -108,9 +118,13 @@ RN_DEV uint32_t permute_oct(uint32_t x, uint32_t oct) {
 RN_DEV void intersect_children(const uint32_t w[20], const TRay &r, float tmax, uint32_t &ihits, uint32_t &thits) {
 	float px = __uint_as_float(w[0]), py = __uint_as_float(w[1]), pz = __uint_as_float(w[2]);
 	uint32_t e = w[3];
-	float sx = __uint_as_float((e & 0xFFu) << 23);
-	float sy = __uint_as_float(((e >> 8) & 0xFFu) << 23);
-	float sz = __uint_as_float(((e >> 16) & 0xFFu) << 23);
+	// Plane bytes are widened to f16 denormals q * 2^-24 with one v_perm per
+	// two bytes and fed to v_fma_mix_f32 against the slab scale times 2^24:
+	// q * 2^-24 * (b * 2^24) is exactly q * b, so each t is the same single
+	// rounding of q * b + a as with a separate byte-to-float conversion.
+	float sx = __uint_as_float(((e & 0xFFu) + 24u) << 23);
+	float sy = __uint_as_float((((e >> 8) & 0xFFu) + 24u) << 23);
+	float sz = __uint_as_float((((e >> 16) & 0xFFu) + 24u) << 23);
 	uint32_t imask = e >> 24;
 	float ax = (px - r.o.x) * r.inv.x, ay = (py - r.o.y) * r.inv.y, az = (pz - r.o.z) * r.inv.z;
 	float bx = sx * r.inv.x, by = sy * r.inv.y, bz = sz * r.inv.z;
@@ -126,24 +140,30 @@ RN_DEV void intersect_children(const uint32_t w[20], const TRay &r, float tmax, 
 	// acc bits 0-23: leaf triangles hit; bits 24-31: slots hit (slot order)
 	uint32_t acc = 0;
 #pragma unroll
-	for (int s = 0; s < 8; s++) {
-		const int sh = (s & 3) * 8;
-		uint32_t nxw = s < 4 ? nx0 : nx1, fxw = s < 4 ? fx0 : fx1;
-		uint32_t nyw = s < 4 ? ny0 : ny1, fyw = s < 4 ? fy0 : fy1;
-		uint32_t nzw = s < 4 ? nz0 : nz1, fzw = s < 4 ? fz0 : fz1;
-		float tnx = fmaf((float)((nxw >> sh) & 0xFFu), bx, ax);
-		float tny = fmaf((float)((nyw >> sh) & 0xFFu), by, ay);
-		float tnz = fmaf((float)((nzw >> sh) & 0xFFu), bz, az);
-		float tfx = fmaf((float)((fxw >> sh) & 0xFFu), bx, ax);
-		float tfy = fmaf((float)((fyw >> sh) & 0xFFu), by, ay);
-		float tfz = fmaf((float)((fzw >> sh) & 0xFFu), bz, az);
-		float tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, 0.0f));
-		float tf = fminf(fminf(tfx, tfy), fminf(tfz, tfar_lim));
-		bool hit = tn <= tf * 1.0000005f;
-		const uint32_t mw = s < 4 ? w[6] : w[7];
-		const uint32_t unary = (mw >> (sh + 5)) & 7u;
-		const uint32_t bits = (unary << ((mw >> sh) & 31u)) | (1u << (24 + s));
-		acc |= hit ? bits : 0u;
+	for (int h = 0; h < 4; h++) { // two slots per pass: 2h and 2h + 1
+		const int wi = h >> 1;
+		const uint32_t sel = (h & 1) ? 0x0C030C02u : 0x0C010C00u;
+		const h16x2 qnx = h16_pair(wi ? nx1 : nx0, sel), qfx = h16_pair(wi ? fx1 : fx0, sel);
+		const h16x2 qny = h16_pair(wi ? ny1 : ny0, sel), qfy = h16_pair(wi ? fy1 : fy0, sel);
+		const h16x2 qnz = h16_pair(wi ? nz1 : nz0, sel), qfz = h16_pair(wi ? fz1 : fz0, sel);
+#pragma unroll
+		for (int k = 0; k < 2; k++) {
+			const int s = 2 * h + k;
+			const int sh = (s & 3) * 8;
+			float tnx = fmaf((float)qnx[k], bx, ax);
+			float tny = fmaf((float)qny[k], by, ay);
+			float tnz = fmaf((float)qnz[k], bz, az);
+			float tfx = fmaf((float)qfx[k], bx, ax);
+			float tfy = fmaf((float)qfy[k], by, ay);
+			float tfz = fmaf((float)qfz[k], bz, az);
+			float tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, 0.0f));
+			float tf = fminf(fminf(tfx, tfy), fminf(tfz, tfar_lim));
+			bool hit = tn <= tf * 1.0000005f;
+			const uint32_t mw = s < 4 ? w[6] : w[7];
+			const uint32_t unary = (mw >> (sh + 5)) & 7u;
+			const uint32_t bits = (unary << ((mw >> sh) & 31u)) | (1u << (24 + s));
+			acc |= hit ? bits : 0u;
+		}
 	}
 	ihits = permute_oct((acc >> 24) & imask, r.oct);
 	thits = acc & 0xFFFFFFu;

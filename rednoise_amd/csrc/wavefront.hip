@@ -73,6 +73,11 @@ struct WQ {
 	WCtl *ctl;
 	const WScene *scenes;
 	int nscenes;
+	// a one-scene render passes its BVH, triangles and frame buffer here so
+	// the trace kernels hold them in scalar registers
+	const Bvh8Node *nodes0;
+	const float4 *tris0;
+	float *out0;
 	unsigned capacity;
 	DevCounters *cnt;
 };
@@ -184,7 +189,7 @@ static_assert(kRefillMin >= 1 && kRefillMin <= 64, "refill threshold is a lane c
 // ballots and on values read from lane 0, so it is uniform over the wave and
 // all 64 lanes reach each __ballot/__shfl together. The loop ends once the
 // chunk counter has passed the queue length and every lane is idle.
-template <int QUERY, int LDS, int SPILL, int BS>
+template <int QUERY, int LDS, int SPILL, int BS, bool ONE>
 RN_DEV void trace_queue(const WQ &q, int cur, uint64_t *stk) {
 	WCtl *c = q.ctl;
 	const unsigned n = QUERY == 0 ? c->cur_count + c->gen_count : c->sh_count;
@@ -201,8 +206,8 @@ RN_DEV void trace_queue(const WQ &q, int cur, uint64_t *stk) {
 	unsigned next = 0, end = 0;
 	bool more = true, valid = false;
 	unsigned idx = 0;
-	const Bvh8Node *nodes = nullptr;
-	const float4 *tris = nullptr;
+	const Bvh8Node *nodes = ONE ? q.nodes0 : nullptr;
+	const float4 *tris = ONE ? q.tris0 : nullptr;
 	Trav<SPILL> t;
 	unsigned long long nrays = 0;
 	while (true) {
@@ -238,12 +243,15 @@ RN_DEV void trace_queue(const WQ &q, int cur, uint64_t *stk) {
 			if (next < end) {
 				const unsigned my = next + (unsigned)__popcll(idle & lt);
 				if (!valid && my < end) {
-					const uint32_t meta = __float_as_uint(M[my].w);
+					// the entry's loads are issued together
+					const float4 o4 = O[my], d4 = D[my];
+					const uint32_t meta = (ONE && QUERY == 1) ? 0u : __float_as_uint(M[my].w);
 					if (QUERY == 1 || (meta >> 16) != kDead) {
-						const DevScene &sc = q.scenes[meta & 0xFFFFu].sc;
-						nodes = sc.nodes;
-						tris = sc.tris;
-						const float4 o4 = O[my], d4 = D[my];
+						if (!ONE) {
+							const DevScene &sc = q.scenes[meta & 0xFFFFu].sc;
+							nodes = sc.nodes;
+							tris = sc.tris;
+						}
 						trav_init<0>(t, xyz(o4), xyz(d4), QUERY == 0 ? INFINITY : d4.w);
 						valid = true;
 						idx = my;
@@ -264,8 +272,8 @@ RN_DEV void trace_queue(const WQ &q, int cur, uint64_t *stk) {
 				q.hit[idx] = make_uint4((uint32_t)t.h.prim, t.h.leaf, __float_as_uint(t.h.t), 0u);
 			} else if (t.h.prim == 0) {
 				const float4 c4 = q.sc[idx];
-				const WScene &ws = q.scenes[__float_as_uint(c4.w)];
-				float *fb = ws.out + 3 * (size_t)__float_as_uint(q.so[idx].w);
+				float *out = ONE ? q.out0 : q.scenes[__float_as_uint(c4.w)].out;
+				float *fb = out + 3 * (size_t)__float_as_uint(q.so[idx].w);
 				atomicAdd(fb, c4.x);
 				atomicAdd(fb + 1, c4.y);
 				atomicAdd(fb + 2, c4.z);
@@ -287,16 +295,16 @@ RN_DEV void trace_queue(const WQ &q, int cur, uint64_t *stk) {
 #define RN_TRACE_ATTR
 #endif
 
-template <int LDS, int SPILL, int BS>
+template <int LDS, int SPILL, int BS, bool ONE>
 __global__ __launch_bounds__(BS) RN_TRACE_ATTR void k_extend(WQ q, int cur) {
 	__shared__ uint64_t stk[LDS * BS];
-	trace_queue<0, LDS, SPILL, BS>(q, cur, stk);
+	trace_queue<0, LDS, SPILL, BS, ONE>(q, cur, stk);
 }
 
-template <int LDS, int SPILL, int BS>
+template <int LDS, int SPILL, int BS, bool ONE>
 __global__ __launch_bounds__(BS) RN_TRACE_ATTR void k_shadow(WQ q) {
 	__shared__ uint64_t stk[LDS * BS];
-	trace_queue<1, LDS, SPILL, BS>(q, 0, stk);
+	trace_queue<1, LDS, SPILL, BS, ONE>(q, 0, stk);
 }
 
 // Shade every queued path at its closest hit: emission (camera rays only),
@@ -304,7 +312,10 @@ __global__ __launch_bounds__(BS) RN_TRACE_ATTR void k_shadow(WQ q) {
 // the continuing path. Pushes are counted per block: one atomic per queue per
 // 1024 entries (same-address atomics serialise at about 11 ns each on
 // MI355X, so one per wave would bound this kernel).
-constexpr int kShadeBlock = 1024;
+#ifndef RN_SHADE_BLOCK
+#define RN_SHADE_BLOCK 1024
+#endif
+constexpr int kShadeBlock = RN_SHADE_BLOCK;
 constexpr int kShadeWaves = kShadeBlock / 64;
 
 __global__ __launch_bounds__(kShadeBlock) void k_shade(WQ q, int cur) {
@@ -526,6 +537,9 @@ void wavefront_render_on(const WaveItem *items, int count, const WaveConfig &cfg
 	RN_HIP(hipMemcpyAsync(dsc, hs.data(), hs.size() * sizeof(WScene), hipMemcpyHostToDevice, s));
 	q.scenes = dsc;
 	q.nscenes = count;
+	q.nodes0 = hs[0].sc.nodes;
+	q.tris0 = hs[0].sc.tris;
+	q.out0 = hs[0].out;
 	q.capacity = cap;
 	q.cnt = cnt;
 
@@ -554,23 +568,30 @@ void wavefront_render_on(const WaveItem *items, int count, const WaveConfig &cfg
 	const unsigned chunks = (cap + kChunk - 1) / kChunk, waves_per_block = (unsigned)bs / 64u;
 	grid = std::min<unsigned>(grid, (chunks + waves_per_block - 1) / waves_per_block);
 	const unsigned sgrid = std::min<unsigned>((unsigned)std::max(1, prop.multiProcessorCount) * 8u, (cap + 255) / 256);
-	const unsigned shgrid = std::min<unsigned>((unsigned)std::max(1, prop.multiProcessorCount) * 2u,
+	const unsigned shgrid = std::min<unsigned>((unsigned)std::max(1, prop.multiProcessorCount) * (2048u / kShadeBlock),
 	                                           (cap + kShadeBlock - 1) / kShadeBlock);
 	auto iteration = [&](int cur) {
 		mark(1, true);
 		k_generate<<<sgrid, 256, 0, s>>>(q, cur);
 		mark(1, false);
 		mark(0, true);
-#define RN_LAUNCH(K, ...)                                                                                             \
+#define RN_LAUNCH1(K, ONE, ...)                                                                                       \
 	do {                                                                                                               \
 		if (deep && bs == 128)                                                                                         \
-			K<kLdsDepth, kSpillDepth, 128><<<grid, 128, 0, s>>>(__VA_ARGS__);                                          \
+			K<kLdsDepth, kSpillDepth, 128, ONE><<<grid, 128, 0, s>>>(__VA_ARGS__);                                     \
 		else if (deep)                                                                                                 \
-			K<kLdsDepth, kSpillDepth, 256><<<grid, 256, 0, s>>>(__VA_ARGS__);                                          \
+			K<kLdsDepth, kSpillDepth, 256, ONE><<<grid, 256, 0, s>>>(__VA_ARGS__);                                     \
 		else if (bs == 128)                                                                                            \
-			K<kLdsDepth, 0, 128><<<grid, 128, 0, s>>>(__VA_ARGS__);                                                    \
+			K<kLdsDepth, 0, 128, ONE><<<grid, 128, 0, s>>>(__VA_ARGS__);                                               \
 		else                                                                                                           \
-			K<kLdsDepth, 0, 256><<<grid, 256, 0, s>>>(__VA_ARGS__);                                                    \
+			K<kLdsDepth, 0, 256, ONE><<<grid, 256, 0, s>>>(__VA_ARGS__);                                               \
+	} while (0)
+#define RN_LAUNCH(K, ...)                                                                                             \
+	do {                                                                                                               \
+		if (count == 1)                                                                                                \
+			RN_LAUNCH1(K, true, __VA_ARGS__);                                                                          \
+		else                                                                                                           \
+			RN_LAUNCH1(K, false, __VA_ARGS__);                                                                         \
 	} while (0)
 		RN_LAUNCH(k_extend, q, cur);
 		mark(0, false);
@@ -581,6 +602,7 @@ void wavefront_render_on(const WaveItem *items, int count, const WaveConfig &cfg
 		RN_LAUNCH(k_shadow, q);
 		mark(2, false);
 #undef RN_LAUNCH
+#undef RN_LAUNCH1
 		k_advance<<<1, 1, 0, s>>>(q, 0);
 	};
 
