@@ -1,6 +1,6 @@
 // GPU LBVH build: centroid bounds -> 30-bit Morton codes -> stable LSD radix
 // sort (wave64 ballot ranking) -> Karras (HPG 2012) binary radix tree ->
-// bottom-up AABB refit with agent-scope atomics -> top-down collapse into the
+// bottom-up AABB refit (LDS inside 1024-leaf chunks, agent-scope atomics above) -> top-down collapse into the
 // compressed 8-wide BVH of rn_types.h (greedy largest-area opening until nodes are
 // full, leaves of <= 3 triangles, octant-ordered child slots, 8-bit
 // conservative boxes).
@@ -39,7 +39,13 @@ __global__ void k_fill_u32(uint32_t *p, uint32_t v, size_t n) {
 		p[i] = v;
 }
 
+// Bounds words are kBoundStride apart (one 128-byte line each): atomics on
+// one line serialise, so six bounds in one line would queue every block's
+// updates behind each other.
+constexpr int kBoundStride = 32;
+
 __global__ void k_centroid_bounds(const float *v, int n, uint32_t *b) {
+	__shared__ float red[6][kBlock / 64];
 	float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
 	for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
 		float3 c = centroid(v, i);
@@ -57,11 +63,23 @@ __global__ void k_centroid_bounds(const float *v, int n, uint32_t *b) {
 			hi[k] = fmaxf(hi[k], __shfl_xor(hi[k], off));
 		}
 	}
+	const int wave = threadIdx.x >> 6;
 	if ((threadIdx.x & 63) == 0) {
 		for (int k = 0; k < 3; k++) {
-			atomicMin(&b[k], ford(lo[k]));
-			atomicMax(&b[3 + k], ford(hi[k]));
+			red[k][wave] = lo[k];
+			red[3 + k][wave] = hi[k];
 		}
+	}
+	__syncthreads();
+	if (threadIdx.x < 6) { // one atomic per block and bound
+		const int k = threadIdx.x;
+		float r = red[k][0];
+		for (int w = 1; w < kBlock / 64; w++)
+			r = k < 3 ? fminf(r, red[k][w]) : fmaxf(r, red[k][w]);
+		if (k < 3)
+			atomicMin(&b[k * kBoundStride], ford(r));
+		else
+			atomicMax(&b[k * kBoundStride], ford(r));
 	}
 }
 
@@ -82,7 +100,7 @@ __global__ void k_morton(const float *v, int n, const uint32_t *b, uint32_t *key
 	uint32_t q[3];
 #pragma unroll
 	for (int k = 0; k < 3; k++) {
-		float lo = fordinv(b[k]), hi = fordinv(b[3 + k]);
+		float lo = fordinv(b[k * kBoundStride]), hi = fordinv(b[(3 + k) * kBoundStride]);
 		float ext = hi - lo;
 		float scale = ext > 0.0f ? 1024.0f / ext : 0.0f;
 		float f = (get3(c, k) - lo) * scale;
@@ -276,22 +294,66 @@ __global__ void k_karras(const uint32_t *k, int n, int *left, int *right, int *p
 		parent[0] = -1;
 }
 
-__global__ void k_refit(const float *v, const uint32_t *order, int n, const int *parent, const int *left,
-                        const int *right, float4 *box, uint32_t *visit) {
-	int p = blockIdx.x * blockDim.x + threadIdx.x;
-	if (p >= n)
+// Bottom-up refit. Block b owns leaves [b*kRefitChunk, (b+1)*kRefitChunk).
+// An internal node whose leaf range lies inside one chunk is "local": Karras
+// node i covers a range that contains i, so it sits at LDS slot i - base, and
+// its children are local too. Local nodes are refitted in LDS with
+// workgroup-scope counters (the second child to arrive computes the parent).
+// Only nodes whose range crosses a chunk boundary (about two per chunk) use
+// agent-scope acquire-release counters in global memory; those are costly on
+// MI355X because agent scope spans eight separately cached XCDs.
+constexpr int kRefitChunk = 1024;
+
+__global__ __launch_bounds__(kRefitChunk) void k_refit(const float *v, const uint32_t *order, int n,
+                                                       const int *parent, const int *left, const int *right,
+                                                       const int2 *range, float4 *box, uint32_t *visit) {
+	__shared__ uint32_t cnt[kRefitChunk];
+	__shared__ float4 lbox[2 * kRefitChunk]; // local internal node boxes (lo, hi)
+	__shared__ float4 leafb[2 * kRefitChunk];
+	const int base = blockIdx.x * kRefitChunk;
+	const int end = min(base + kRefitChunk, n);
+	const int p = base + (int)threadIdx.x;
+	cnt[threadIdx.x] = 0u;
+	float4 lo = make_float4(0.f, 0.f, 0.f, 0.f), hi = lo;
+	if (p < n) {
+		const float *t = v + 9 * (size_t)order[p];
+		lo = make_float4(fminf(fminf(t[0], t[3]), t[6]), fminf(fminf(t[1], t[4]), t[7]),
+		                 fminf(fminf(t[2], t[5]), t[8]), 0.0f);
+		hi = make_float4(fmaxf(fmaxf(t[0], t[3]), t[6]), fmaxf(fmaxf(t[1], t[4]), t[7]),
+		                 fmaxf(fmaxf(t[2], t[5]), t[8]), 0.0f);
+		size_t li = (size_t)(n - 1 + p);
+		box[2 * li] = lo;
+		box[2 * li + 1] = hi;
+		leafb[2 * threadIdx.x] = lo;
+		leafb[2 * threadIdx.x + 1] = hi;
+	}
+	__syncthreads();
+	if (p >= n || n < 2)
 		return;
-	const float *t = v + 9 * (size_t)order[p];
-	float4 lo = make_float4(fminf(fminf(t[0], t[3]), t[6]), fminf(fminf(t[1], t[4]), t[7]),
-	                        fminf(fminf(t[2], t[5]), t[8]), 0.0f);
-	float4 hi = make_float4(fmaxf(fmaxf(t[0], t[3]), t[6]), fmaxf(fmaxf(t[1], t[4]), t[7]),
-	                        fmaxf(fmaxf(t[2], t[5]), t[8]), 0.0f);
-	size_t li = (size_t)(n - 1 + p);
-	box[2 * li] = lo;
-	box[2 * li + 1] = hi;
-	if (n < 2)
-		return;
-	int node = parent[li];
+	int node = parent[n - 1 + p];
+	// local part
+	while (node >= 0) {
+		const int2 rg = range[node];
+		if (rg.x < base || rg.y >= end)
+			break;
+		const int slot = node - base;
+		uint32_t old = __hip_atomic_fetch_add(&cnt[slot], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+		if (old == 0)
+			return;
+		const int l = left[node], r = right[node];
+		const float4 *ca = l >= 0 ? &lbox[2 * (l - base)] : &leafb[2 * (~l - base)];
+		const float4 *cb = r >= 0 ? &lbox[2 * (r - base)] : &leafb[2 * (~r - base)];
+		const float4 al = ca[0], ah = ca[1], bl = cb[0], bh = cb[1];
+		lo = make_float4(fminf(al.x, bl.x), fminf(al.y, bl.y), fminf(al.z, bl.z), 0.0f);
+		hi = make_float4(fmaxf(ah.x, bh.x), fmaxf(ah.y, bh.y), fmaxf(ah.z, bh.z), 0.0f);
+		lbox[2 * slot] = lo;
+		lbox[2 * slot + 1] = hi;
+		box[2 * (size_t)node] = lo;
+		box[2 * (size_t)node + 1] = hi;
+		node = parent[node];
+	}
+	// nodes spanning chunks: the box just written is published by the
+	// release half of the agent-scope counter update
 	while (node >= 0) {
 		uint32_t old = __hip_atomic_fetch_add(&visit[node], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
 		if (old == 0)
@@ -526,13 +588,13 @@ void fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s) {
 
 // Codes + sorted order + (n >= 2) binary tree and refitted boxes.
 void lbvh_front(Bvh &b, const float *d_verts, int n, hipStream_t s) {
-	uint32_t *bounds = static_cast<uint32_t *>(b.bounds.reserve(8 * sizeof(uint32_t)));
+	uint32_t *bounds = static_cast<uint32_t *>(b.bounds.reserve(6 * kBoundStride * sizeof(uint32_t)));
 	uint32_t *keys = static_cast<uint32_t *>(b.keys.reserve((size_t)n * 4));
 	uint32_t *vals = static_cast<uint32_t *>(b.vals.reserve((size_t)n * 4));
 	uint32_t *keys2 = static_cast<uint32_t *>(b.keys2.reserve((size_t)n * 4));
 	uint32_t *vals2 = static_cast<uint32_t *>(b.vals2.reserve((size_t)n * 4));
-	fill_u32(bounds, 0xFFFFFFFFu, 3, s);
-	fill_u32(bounds + 3, 0u, 3, s);
+	fill_u32(bounds, 0xFFFFFFFFu, 3 * kBoundStride, s);
+	fill_u32(bounds + 3 * kBoundStride, 0u, 3 * kBoundStride, s);
 	k_centroid_bounds<<<std::min(1024u, grid_for(n)), kBlock, 0, s>>>(d_verts, n, bounds);
 	k_morton<<<grid_for(n), kBlock, 0, s>>>(d_verts, n, bounds, keys, vals);
 	radix_sort_pairs(keys, vals, keys2, vals2, n, 30, b.hist, b.scan_tmp, s);
@@ -547,7 +609,8 @@ void lbvh_front(Bvh &b, const float *d_verts, int n, hipStream_t s) {
 		k_karras<<<grid_for(n - 1), kBlock, 0, s>>>(keys, n, left, right, parent, range);
 		fill_u32(visit, 0u, (size_t)(n - 1), s);
 	}
-	k_refit<<<grid_for(n), kBlock, 0, s>>>(d_verts, vals, n, parent, left, right, box, visit);
+	k_refit<<<grid_for(n, kRefitChunk), kRefitChunk, 0, s>>>(d_verts, vals, n, parent, left, right, range, box,
+	                                                         visit);
 }
 
 } // namespace
