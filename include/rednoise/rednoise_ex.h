@@ -215,6 +215,13 @@ int rn_philox4x32_10(int n, const uint32_t *ctr, const uint32_t *key, uint32_t *
 /* Name of the GPU the library runs on (e.g. "gfx950"), or "" if none. */
 const char *rn_device_name(void);
 
+/* Frees the device work buffers the library keeps between calls (wavefront
+ * path and shadow queues, staging for host outputs). The wavefront keeps up
+ * to 2^27 paths in flight by default (about 21 GB on a large GPU; see
+ * rn_render_params.queue_paths); the next render allocates again. Scenes and
+ * their BVHs are untouched. Returns 1, or 0 with rn_last_error() set. */
+int rn_release_workspace(void);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif

@@ -171,7 +171,7 @@ def _configure(lib: C.CDLL) -> None:
     sig = {
         "rn_scene_load_obj": ([cp, f], vp), "rn_scene_free": ([vp], None), "rn_scene_triangle_count": ([vp], i),
         "rn_render": ([vp, i, i, i, f, i, vp], i), "rn_render_seeded": ([vp, i, i, f, i, C.c_uint32, C.c_uint32, vp], i), "rn_save_png": ([cp, i, i, vp], i), "rn_version": ([], cp),
-        "rn_last_error": ([], cp), "rn_device_name": ([], cp),
+        "rn_last_error": ([], cp), "rn_device_name": ([], cp), "rn_release_workspace": ([], i),
         "rn_scene_from_arrays": ([vp, i, vp, vp], vp), "rn_scene_set_materials": ([vp, vp, vp], i),
         "rn_scene_get_triangles": ([vp, vp, vp, vp], i), "rn_scene_get_compat": ([vp, vp, vp, vp, vp, vp], i),
         "rn_scene_get_texture": ([vp, i, vp, vp, vp], i),
@@ -209,6 +209,15 @@ def version() -> str:
 
 def device_name() -> str:
     return _load_library().rn_device_name().decode()
+
+
+def release_workspace() -> None:
+    """Free the device work buffers kept between renders (the wavefront keeps
+    up to 2^27 paths in flight by default, about 21 GB); the next render
+    allocates them again."""
+    lib = _load_library()
+    if lib.rn_release_workspace() != 1:
+        raise RuntimeError(lib.rn_last_error().decode())
 
 
 def last_stats() -> dict:

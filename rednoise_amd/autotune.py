@@ -18,7 +18,7 @@ from typing import List, Optional, Sequence, Tuple
 
 from . import BatchOptions, Camera, RenderParams, Scene, device_name, last_stats, render_batch
 
-QUEUE_SIZES = (1 << 20, 1 << 21, 1 << 22)
+QUEUE_SIZES = (1 << 23, 1 << 25, 1 << 27)
 BVH_WIDTHS = (8, 4)
 BLOCKS = (256, 128)
 

@@ -340,6 +340,21 @@ const char *rn_last_error(void) { return t_error.c_str(); }
 
 const char *rn_version(void) { return "0.1.2"; }
 
+int rn_release_workspace(void) {
+	RN_TRY
+	t_error.clear();
+	Context &c = ctx();
+	std::lock_guard<std::mutex> lk(c.mu);
+	if (c.stream)
+		RN_HIP(hipStreamSynchronize(c.stream));
+	RN_HIP(hipDeviceSynchronize());
+	wavefront_release();
+	for (DevBuf *b : {&c.out, &c.in0, &c.in1, &c.in2, &c.in3, &c.o0, &c.o1, &c.o2, &c.o3, &c.o4})
+		b->release();
+	return 1;
+	RN_CATCH(0)
+}
+
 const char *rn_device_name(void) {
 	RN_TRY
 	static std::string name;

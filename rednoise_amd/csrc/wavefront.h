@@ -15,7 +15,7 @@ struct WaveItem {
 };
 
 struct WaveConfig {
-	int queue_paths = 0;     // <= 0: min(total paths, 2^21)
+	int queue_paths = 0;     // <= 0: min(total paths, 2^27, free memory / 4 / 160 B)
 	int block = 0;           // <= 0: 256
 	bool use_graph = false;  // capture the iteration loop in a hipGraph
 	int iters_per_graph = 0; // <= 0: 8 (rounded up to even)
@@ -33,5 +33,8 @@ void wavefront_render(const WaveItem *items, int count, const WaveConfig &cfg, D
                       WaveStats &st);
 
 void fill_zero(void *p, size_t bytes, hipStream_t s);
+
+// Frees the wavefront queues (they are reallocated by the next render).
+void wavefront_release();
 
 } // namespace rn

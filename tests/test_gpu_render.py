@@ -138,3 +138,18 @@ def test_autotuner_picks_and_caches(rn, tmp_path, monkeypatch):
     for (sc, cam, p), img in zip(items, imgs):
         p.bvh_width = 8
         assert np.allclose(img, sc.render_hdr(cam, p), rtol=1e-4, atol=1e-5)
+
+
+def test_release_workspace_then_render_again(rn):
+    """Freeing the cached work buffers between renders changes nothing; the
+    default (large) queue and a small explicit queue give the same image."""
+    sc, cam, p = _items(rn)[0]
+    p.integrator = rn.INTEGRATOR_WAVEFRONT
+    p.queue_paths = 0
+    a = sc.render_hdr(cam, p)
+    rn.release_workspace()
+    b = sc.render_hdr(cam, p)
+    p.queue_paths = 4096
+    c = sc.render_hdr(cam, p)
+    assert np.allclose(a, b, rtol=1e-5, atol=1e-6)
+    assert np.allclose(a, c, rtol=1e-4, atol=1e-5)
