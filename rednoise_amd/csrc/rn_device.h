@@ -394,11 +394,13 @@ RN_DEV bool trav_step(Trav<SPILL> &s, const Bvh8Node *__restrict__ nodes, const 
 			uint32_t k = __builtin_ctz(m);
 			m &= m - 1u;
 			uint32_t ti = s.tri_base + k;
-			float4 a = gload_f4(tris, 3 * ti), b4 = gload_f4(tris, 3 * ti + 1), c4 = gload_f4(tris, 3 * ti + 2);
-			int prim = __float_as_int(a.w);
-			if (prim == ignore)
-				continue;
-			if (COUNT)
+			// The id test does not branch before the triangle test: a branch
+			// there lets the compiler sink the second and third loads behind
+			// the first, which costs a second memory round trip per test.
+			const float4 a = gload_f4(tris, 3 * ti), b4 = gload_f4(tris, 3 * ti + 1), c4 = gload_f4(tris, 3 * ti + 2);
+			const int prim = __float_as_int(a.w);
+			const bool skip = prim == ignore;
+			if (COUNT && !skip)
 				cnt[1]++;
 			float tt, uu, vv;
 			bool ok;
@@ -406,7 +408,7 @@ RN_DEV bool trav_step(Trav<SPILL> &s, const Bvh8Node *__restrict__ nodes, const 
 				ok = tri_watertight(s.wr, s.r.o, xyz(a), xyz(b4), xyz(c4), s.h.t, tt, uu, vv);
 			else
 				ok = tri_compat(s.r.o, s.r.d, xyz(a), xyz(b4), xyz(c4), tt, uu, vv) && tt < s.h.t;
-			if (ok) {
+			if (ok && !skip) {
 				if (QUERY == 1) {
 					s.h.prim = 1;
 					return true;
