@@ -278,6 +278,7 @@ struct Hit {
 	int prim;      // original triangle id, -1 = miss (any-hit: 1 = occluded)
 	uint32_t leaf; // leaf-ordered triangle index of the hit
 	float t, u, v;
+	float3 ng;     // with NG: cross(v1 - v0, v2 - v0) of the hit triangle
 };
 
 // Scene buffers are always global memory; loading through address space 1
@@ -358,7 +359,7 @@ RN_DEV void trav_init(Trav<SPILL> &s, float3 o, float3 d, float tmax) {
 #endif
 constexpr int kTriBudget = RN_TRI_BUDGET;
 
-template <int QUERY, int TRI, int LDS_DEPTH, int SPILL, bool COUNT = false>
+template <int QUERY, int TRI, int LDS_DEPTH, int SPILL, bool COUNT = false, bool NG = false>
 RN_DEV bool trav_step(Trav<SPILL> &s, const Bvh8Node *__restrict__ nodes, const float4 *__restrict__ tris, int ignore,
                       uint64_t *lds, int ldsStride, int lane, uint32_t *cnt = nullptr) {
 	uint32_t hits = (uint32_t)(s.grp >> 40);
@@ -418,6 +419,8 @@ RN_DEV bool trav_step(Trav<SPILL> &s, const Bvh8Node *__restrict__ nodes, const 
 				s.h.t = tt;
 				s.h.u = uu;
 				s.h.v = vv;
+				if (NG) // the vertices are in registers now; the shader needs no reload
+					s.h.ng = cross3(xyz(b4) - xyz(a), xyz(c4) - xyz(a));
 			}
 		}
 		s.thits = m;

@@ -50,10 +50,15 @@ RN_DEV int pick_light(const float *cdf, int nl, float u) {
 RN_DEV float3 load3(const float *p, int i) { return f3(p[3 * i], p[3 * i + 1], p[3 * i + 2]); }
 
 // Geometric normal of leaf-ordered triangle `leaf`, facing against d.
+// Unit geometric normal facing against d, from cross(v1 - v0, v2 - v0).
+RN_DEV float3 facing_from_cross(float3 c, float3 d) {
+	float3 ng = normalize3(c);
+	return dot3(ng, d) > 0.0f ? ng * -1.0f : ng;
+}
+
 RN_DEV float3 facing_normal(const float4 *tris, uint32_t leaf, float3 d) {
 	float4 a = tris[3 * leaf], b = tris[3 * leaf + 1], c = tris[3 * leaf + 2];
-	float3 ng = normalize3(cross3(xyz(b) - xyz(a), xyz(c) - xyz(a)));
-	return dot3(ng, d) > 0.0f ? ng * -1.0f : ng;
+	return facing_from_cross(cross3(xyz(b) - xyz(a), xyz(c) - xyz(a)), d);
 }
 
 // Light sample for NEE from shading origin `so` with normal ng. Returns the

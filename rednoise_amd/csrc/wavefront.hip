@@ -68,7 +68,8 @@ RN_DEV void reset_stripes(WCtl *c) {
 
 struct WQ {
 	float4 *o[2], *d[2], *b[2];
-	uint4 *hit; // per queue entry: prim (-1 = miss), leaf index, t
+	int *hitp;     // per queue entry: hit triangle id, -1 = miss
+	float4 *hitn;  // per queue entry: cross(v1 - v0, v2 - v0) of the hit triangle, t
 	float4 *so, *sd, *sc;
 	WCtl *ctl;
 	const WScene *scenes;
@@ -266,10 +267,12 @@ RN_DEV void trace_queue(const WQ &q, int cur, uint64_t *stk) {
 				break;
 			continue;
 		}
-		if (valid && trav_step<QUERY, 0, LDS, SPILL>(t, nodes, tris, -1, stk, BS, threadIdx.x)) {
+		if (valid && trav_step<QUERY, 0, LDS, SPILL, false, QUERY == 0>(t, nodes, tris, -1, stk, BS, threadIdx.x)) {
 			valid = false;
 			if (QUERY == 0) {
-				q.hit[idx] = make_uint4((uint32_t)t.h.prim, t.h.leaf, __float_as_uint(t.h.t), 0u);
+				q.hitp[idx] = t.h.prim;
+				if (t.h.prim >= 0)
+					q.hitn[idx] = make_float4(t.h.ng.x, t.h.ng.y, t.h.ng.z, t.h.t);
 			} else if (t.h.prim == 0) {
 				const float4 c4 = q.sc[idx];
 				float *out = ONE ? q.out0 : q.scenes[__float_as_uint(c4.w)].out;
@@ -318,6 +321,7 @@ __global__ __launch_bounds__(BS) RN_TRACE_ATTR void k_shadow(WQ q) {
 constexpr int kShadeBlock = RN_SHADE_BLOCK;
 constexpr int kShadeWaves = kShadeBlock / 64;
 
+template <bool ONE>
 __global__ __launch_bounds__(kShadeBlock) void k_shade(WQ q, int cur) {
 	__shared__ unsigned wcnt[2][kShadeWaves];
 	__shared__ unsigned bbase[2];
@@ -341,19 +345,20 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(WQ q, int cur) {
 		float shmax = 0.f;
 		float4 o4 = make_float4(0.f, 0.f, 0.f, 0.f), d4 = o4;
 		if (depth != kDead) {
-			const uint4 hr = q.hit[i];
-			const int prim = (int)hr.x;
+			const int prim = q.hitp[i];
 			if (prim >= 0) {
+				const float4 hn = q.hitn[i];
 				o4 = O[i];
 				d4 = D[i];
 				beta = xyz(b4);
-				const WScene &ws = q.scenes[meta & 0xFFFFu];
+				// one scene: a uniform address, loaded once into scalar registers
+				const WScene &ws = q.scenes[ONE ? 0u : (meta & 0xFFFFu)];
 				const DevScene &sc = ws.sc;
 				const DevParams &p = ws.p;
 				float3 o = xyz(o4), d = xyz(d4);
 				uint32_t pix = __float_as_uint(o4.w), smp = __float_as_uint(d4.w);
-				float3 P = o + d * __uint_as_float(hr.z);
-				float3 ng = facing_normal(sc.tris, hr.y, d);
+				float3 P = o + d * hn.w;
+				float3 ng = facing_from_cross(xyz(hn), d);
 				if (depth == 0) {
 					float3 le = load3(sc.emission, prim);
 					if (le.x != 0.f || le.y != 0.f || le.z != 0.f) {
@@ -528,7 +533,7 @@ void wavefront_render_on(const WaveItem *items, int count, const WaveConfig &cfg
 	// in flight, 81 ms with 16M and 71 ms with the whole frame (133M) in
 	// flight. By default up to 2^27 paths (160 bytes each, 21 GB) are kept,
 	// but no more than a quarter of the free device memory.
-	constexpr size_t kEntryBytes = 10 * sizeof(float4); // path state 2 x 3, shadow ray 3, hit 1
+	constexpr size_t kEntryBytes = 10 * sizeof(float4) + sizeof(int); // path state 2 x 3, shadow ray 3, hit
 	unsigned long long want;
 	if (cfg.queue_paths > 0) {
 		want = std::min<unsigned long long>((unsigned long long)cfg.queue_paths, 1ull << 30);
@@ -548,7 +553,9 @@ void wavefront_render_on(const WaveItem *items, int count, const WaveConfig &cfg
 	q.o[1] = st4 + 3 * (size_t)cap;
 	q.d[1] = st4 + 4 * (size_t)cap;
 	q.b[1] = st4 + 5 * (size_t)cap;
-	q.hit = static_cast<uint4 *>(ww.hit.reserve((size_t)cap * sizeof(uint4)));
+	float4 *hit4 = static_cast<float4 *>(ww.hit.reserve((size_t)cap * (sizeof(float4) + sizeof(int))));
+	q.hitn = hit4;
+	q.hitp = reinterpret_cast<int *>(hit4 + cap);
 	float4 *sh4 = static_cast<float4 *>(ww.shadow.reserve((size_t)cap * 3 * sizeof(float4)));
 	q.so = sh4;
 	q.sd = sh4 + cap;
@@ -617,7 +624,10 @@ void wavefront_render_on(const WaveItem *items, int count, const WaveConfig &cfg
 		RN_LAUNCH(k_extend, q, cur);
 		mark(0, false);
 		mark(1, true);
-		k_shade<<<shgrid, kShadeBlock, 0, s>>>(q, cur);
+		if (count == 1)
+			k_shade<true><<<shgrid, kShadeBlock, 0, s>>>(q, cur);
+		else
+			k_shade<false><<<shgrid, kShadeBlock, 0, s>>>(q, cur);
 		mark(1, false);
 		mark(2, true);
 		RN_LAUNCH(k_shadow, q);
