@@ -191,6 +191,62 @@ def batch_bench(rn, torch):
             "scene_generation_s": round(gen_s, 1)}
 
 
+def terrain_bench(rn, lib, torch):
+    """Config 4: the 9,990,450-triangle terrain with 8 quad lights at
+    3840x2160, 256 spp, depth 12, RR from depth 3: one step = GPU BVH build +
+    wavefront render (1 warm-up step, 1 timed). Then the terrain at 64 spp
+    batched with 16 config-2 scenes in one rn_render_batch call, beside the
+    same 17 renders made one call each."""
+    from rednoise_amd import scenes
+    t0 = time.perf_counter()
+    sd = scenes.terrain_scene()
+    gen_s = time.perf_counter() - t0
+    sc = rn.Scene.from_arrays(sd.verts, sd.albedo, sd.emission)
+    cam = rn.Camera.make(**sd.camera)
+    W, H, spp, depth, rr, eps = 3840, 2160, 256, 12, 3, 0.05
+    out = torch.empty(W * H * 3, dtype=torch.float32, device="cuda")
+    p = rn.RenderParams.make(W, H, spp, depth, rr, seed=0, ray_eps=eps, integrator=rn.INTEGRATOR_WAVEFRONT)
+    dt, st = timed_steps(rn, lib, sc, cam, p, out.data_ptr(), 1, 1, torch)
+    rays = st[0]["rays_closest"] + st[0]["rays_shadow"]
+    res = {"workload": f"config4: {sd.notes}, {W}x{H}, {spp} spp, depth {depth}, RR from depth {rr}, "
+                       "LBVH+BVH8 built on the GPU in the step, wavefront",
+           "mrays_s": round(rays / dt / 1e6, 2), "seconds_per_step": round(dt, 3),
+           "build_ms": round(st[0]["build_ms"], 2), "bvh": {"nodes": st[0]["bvh_nodes"], "depth": st[0]["bvh_depth"]},
+           "rays": {"closest": st[0]["rays_closest"], "shadow": st[0]["rays_shadow"], "paths": st[0]["paths"]},
+           "scene_generation_s": round(gen_s, 1)}
+    del out
+    specs = scenes.config2_batch(16, seed=2)
+    pt = rn.RenderParams.make(W, H, 64, depth, rr, seed=0, ray_eps=eps)
+    items = [(sc, cam, pt)]
+    for s in specs:
+        sci = rn.Scene.from_arrays(s.scene.verts, s.scene.albedo, s.scene.emission)
+        items.append((sci, rn.Camera.make(**s.scene.camera),
+                      rn.RenderParams.make(s.width, s.height, s.spp, s.max_depth, 3, seed=0, ray_eps=0.03)))
+    opts = rn.BatchOptions(0, 0, 8, 1, 8)
+    rn.render_batch(items, opts)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rn.render_batch(items, opts)
+    dtb = time.perf_counter() - t0
+    stb = rn.last_stats()
+    rays_b = stb["rays_closest"] + stb["rays_shadow"]
+    t0 = time.perf_counter()
+    rays_s = 0
+    for sci, ci, pi in items:
+        pi.integrator = rn.INTEGRATOR_WAVEFRONT
+        sci.render_hdr(ci, pi)
+        s2 = rn.last_stats()
+        rays_s += s2["rays_closest"] + s2["rays_shadow"]
+    dts = time.perf_counter() - t0
+    res["batch_with_16_config2"] = {
+        "workload": "terrain at 64 spp + 16 config-2 scenes, one rn_render_batch call (hipGraph replay)",
+        "batched_mrays_s": round(rays_b / dtb / 1e6, 2), "batched_s": round(dtb, 3),
+        "one_call_each_mrays_s": round(rays_s / dts / 1e6, 2), "one_call_each_s": round(dts, 3)}
+    for sci, _, _ in items:
+        sci.close()
+    return res
+
+
 def cpu_baseline(sd, W, H, spp, depth, rr, eps, budget_s=12.0):
     """FP32 oracle (C port of the estimator, OpenMP) on a pixel sample of the
     same workload; sample grows until ~budget_s of CPU work."""
@@ -368,6 +424,11 @@ def main():
             log(f"batch: {result['batch']}")
         except Exception as e:
             result["batch"] = {"error": str(e)}
+        try:
+            result["config4"] = terrain_bench(rn, lib, torch)
+            log(f"config4: {result['config4']}")
+        except Exception as e:
+            result["config4"] = {"error": str(e)}
         try:
             r = reference_modeR(rn, torch)
             if r:
