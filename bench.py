@@ -247,6 +247,38 @@ def terrain_bench(rn, lib, torch):
     return res
 
 
+def scene_cache_bench(rn, sd):
+    """The config-1 scene written as an OBJ, then loaded with
+    rn_scene_load_obj (the reference's entry point) and with rn_scene_load
+    from its binary cache (host-side load; the GPU upload is the same)."""
+    import tempfile
+    with tempfile.TemporaryDirectory() as d:
+        obj = os.path.join(d, "config1.obj")
+        v = sd.verts.reshape(-1, 3).astype(np.float64)
+        with open(obj, "w") as f:
+            np.savetxt(f, v, fmt="v %.9g %.9g %.9g")
+            faces = np.arange(1, v.shape[0] + 1, dtype=np.int64).reshape(-1, 3)
+            np.savetxt(f, faces, fmt="f %d %d %d")
+        t0 = time.perf_counter()
+        a = rn.Scene.load_obj(obj, 1.0)
+        t_obj = time.perf_counter() - t0
+        cache = os.path.join(d, "config1.rnscene")
+        t0 = time.perf_counter()
+        a.save(cache)
+        t_save = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        b = rn.Scene.load(cache)
+        t_load = time.perf_counter() - t0
+        same = bool(np.array_equal(a.triangles()[0], b.triangles()[0]))
+        res = {"triangles": a.triangle_count, "obj_mb": round(os.path.getsize(obj) / 1e6, 1),
+               "obj_load_s": round(t_obj, 3), "cache_mb": round(os.path.getsize(cache) / 1e6, 1),
+               "cache_save_s": round(t_save, 3), "cache_load_s": round(t_load, 3),
+               "load_speedup": round(t_obj / t_load, 1), "identical_geometry": same}
+        a.close()
+        b.close()
+    return res
+
+
 def cpu_baseline(sd, W, H, spp, depth, rr, eps, budget_s=12.0):
     """FP32 oracle (C port of the estimator, OpenMP) on a pixel sample of the
     same workload; sample grows until ~budget_s of CPU work."""
@@ -424,6 +456,11 @@ def main():
             log(f"batch: {result['batch']}")
         except Exception as e:
             result["batch"] = {"error": str(e)}
+        try:
+            result["scene_cache"] = scene_cache_bench(rn, sd)
+            log(f"scene_cache: {result['scene_cache']}")
+        except Exception as e:
+            result["scene_cache"] = {"error": str(e)}
         try:
             result["config4"] = terrain_bench(rn, lib, torch)
             log(f"config4: {result['config4']}")

@@ -38,6 +38,18 @@ rn_scene *rn_scene_from_arrays(const float *verts, int n, const float *albedo, c
  * light set does not change. */
 int rn_scene_set_materials(rn_scene *scene, const float *albedo, const float *emission);
 
+/* Binary scene cache. rn_scene_save writes everything a scene holds
+ * (geometry, both material sets, the reference estimator's attributes and
+ * textures) as raw arrays with a checksum; rn_scene_load reads it back, so a
+ * scene parsed once from OBJ (rn_scene_load_obj, src/capi.cpp:22-34 and
+ * src/ObjLoader.cpp:102-214 in the reference) reloads at file-read speed.
+ * A loaded scene renders exactly as the saved one. No reference counterpart:
+ * it replaces re-parsing the OBJ on every start. Save returns 1/0; load
+ * returns NULL on a missing, truncated, corrupt or foreign file, with the
+ * reason in rn_last_error(). */
+int rn_scene_save(const rn_scene *scene, const char *path);
+rn_scene *rn_scene_load(const char *path);
+
 /* Copy the scene's triangle data out: verts 9*n, albedo 3*n, emission 3*n
  * (any may be NULL). For an OBJ scene albedo = Kd colour int / 255. */
 int rn_scene_get_triangles(const rn_scene *scene, float *verts, float *albedo, float *emission);

@@ -172,6 +172,7 @@ def _configure(lib: C.CDLL) -> None:
         "rn_scene_load_obj": ([cp, f], vp), "rn_scene_free": ([vp], None), "rn_scene_triangle_count": ([vp], i),
         "rn_render": ([vp, i, i, i, f, i, vp], i), "rn_render_seeded": ([vp, i, i, f, i, C.c_uint32, C.c_uint32, vp], i), "rn_save_png": ([cp, i, i, vp], i), "rn_version": ([], cp),
         "rn_last_error": ([], cp), "rn_device_name": ([], cp), "rn_release_workspace": ([], i),
+        "rn_scene_save": ([vp, cp], i), "rn_scene_load": ([cp], vp),
         "rn_scene_from_arrays": ([vp, i, vp, vp], vp), "rn_scene_set_materials": ([vp, vp, vp], i),
         "rn_scene_get_triangles": ([vp, vp, vp, vp], i), "rn_scene_get_compat": ([vp, vp, vp, vp, vp, vp], i),
         "rn_scene_get_texture": ([vp, i, vp, vp, vp], i),
@@ -307,6 +308,22 @@ class Scene:
         if not h:
             raise _err(f"rn_scene_load_obj({path!r})")
         return cls(h)
+
+    @classmethod
+    def load(cls, path: Union[str, os.PathLike]) -> "Scene":
+        """Load a binary scene cache written by :meth:`save` (``rn_scene_load``)."""
+        path = os.fspath(path)
+        if not os.path.isfile(path):
+            raise FileNotFoundError(f"scene cache not found: {path!r}")
+        h = _load_library().rn_scene_load(os.fsencode(path))
+        if not h:
+            raise _err(f"rn_scene_load({path!r})")
+        return cls(h)
+
+    def save(self, path: Union[str, os.PathLike]) -> None:
+        """Write this scene as a binary scene cache (``rn_scene_save``)."""
+        if _load_library().rn_scene_save(self.handle, os.fsencode(os.fspath(path))) != 1:
+            raise _err(f"rn_scene_save({os.fspath(path)!r})")
 
     @classmethod
     def from_arrays(cls, verts, albedo, emission=None) -> "Scene":

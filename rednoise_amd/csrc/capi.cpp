@@ -418,6 +418,44 @@ rn_scene *rn_scene_from_arrays(const float *verts, int n, const float *albedo, c
 	RN_CATCH(nullptr)
 }
 
+int rn_scene_save(const rn_scene *scene, const char *path) {
+	RN_TRY
+	t_error.clear();
+	if (!scene || !path) {
+		set_error("scene and path must be non-NULL");
+		return 0;
+	}
+	std::lock_guard<std::mutex> lk(const_cast<rn_scene *>(scene)->mu);
+	std::string err;
+	if (!save_scene(path, scene->h, err)) {
+		set_error(err);
+		return 0;
+	}
+	return 1;
+	RN_CATCH(0)
+}
+
+rn_scene *rn_scene_load(const char *path) {
+	RN_TRY
+	t_error.clear();
+	if (!path) {
+		set_error("path is NULL");
+		return nullptr;
+	}
+	std::unique_ptr<rn_scene> s(new (std::nothrow) rn_scene());
+	if (!s) {
+		set_error("out of memory");
+		return nullptr;
+	}
+	std::string err;
+	if (!load_scene(path, s->h, err)) {
+		set_error(err);
+		return nullptr;
+	}
+	return s.release();
+	RN_CATCH(nullptr)
+}
+
 void rn_scene_free(rn_scene *scene) { delete scene; }
 
 int rn_scene_triangle_count(const rn_scene *scene) { return scene ? scene->h.n : 0; }

@@ -398,4 +398,202 @@ bool write_png(const std::string &path, int w, int h, const unsigned char *rgba)
 	return ok;
 }
 
+// ------------------------------------------------------- binary scene cache --
+//
+// A scene cache file holds a HostScene as raw little-endian arrays so a
+// large scene loads at file-read speed instead of OBJ-parsing speed:
+//   bytes 0-7    "RNSCENE1"
+//   u32          format version (1)
+//   u32          n (triangles)
+//   u32          number of textures
+//   u32          reserved (0)
+//   u64          checksum of everything after this header
+//   f32 verts[9n], f32 albedo[3n], f32 emission[3n], f32 normals[3n],
+//   i32 colour[3n], i32 material[n], i32 texture_id[n], f32 texcoords[6n],
+//   then per texture: i32 width, i32 height, u32 argb[width * height].
+// The checksum folds the payload as 64-bit words (the tail zero-padded):
+// h = rotl(h ^ w, 29) * 0x9E3779B97F4A7C15, starting from the byte count.
+
+namespace {
+
+constexpr char kCacheMagic[8] = {'R', 'N', 'S', 'C', 'E', 'N', 'E', '1'};
+constexpr uint32_t kCacheVersion = 1;
+
+struct CacheHash {
+	uint64_t h = 0, bytes = 0;
+	unsigned char tail[8];
+	int ntail = 0;
+	void word(uint64_t w) {
+		h ^= w;
+		h = (h << 29) | (h >> 35);
+		h *= 0x9E3779B97F4A7C15ull;
+	}
+	void add(const void *p, size_t n) {
+		const unsigned char *b = static_cast<const unsigned char *>(p);
+		bytes += n;
+		while (n > 0 && ntail > 0 && ntail < 8) {
+			tail[ntail++] = *b++;
+			n--;
+			if (ntail == 8) {
+				uint64_t w;
+				std::memcpy(&w, tail, 8);
+				word(w);
+				ntail = 0;
+			}
+		}
+		while (n >= 8) {
+			uint64_t w;
+			std::memcpy(&w, b, 8);
+			word(w);
+			b += 8;
+			n -= 8;
+		}
+		while (n > 0) {
+			tail[ntail++] = *b++;
+			n--;
+		}
+	}
+	uint64_t finish() {
+		if (ntail > 0) {
+			std::memset(tail + ntail, 0, (size_t)(8 - ntail));
+			uint64_t w;
+			std::memcpy(&w, tail, 8);
+			word(w);
+			ntail = 0;
+		}
+		word(bytes);
+		return h;
+	}
+};
+
+template <class T> bool put(std::FILE *f, CacheHash &hs, const std::vector<T> &v) {
+	if (v.empty())
+		return true;
+	hs.add(v.data(), v.size() * sizeof(T));
+	return std::fwrite(v.data(), sizeof(T), v.size(), f) == v.size();
+}
+
+template <class T> bool get(std::FILE *f, CacheHash &hs, std::vector<T> &v, size_t count) {
+	v.resize(count);
+	if (count == 0)
+		return true;
+	if (std::fread(v.data(), sizeof(T), count, f) != count)
+		return false;
+	hs.add(v.data(), count * sizeof(T));
+	return true;
+}
+
+} // namespace
+
+bool save_scene(const std::string &path, const HostScene &sc, std::string &err) {
+	const size_t n = (size_t)sc.n;
+	if (sc.verts.size() != 9 * n || sc.albedo.size() != 3 * n || sc.emission.size() != 3 * n ||
+	    sc.normals.size() != 3 * n || sc.colour.size() != 3 * n || sc.material.size() != n ||
+	    sc.texture_id.size() != n || sc.texcoords.size() != 6 * n) {
+		err = "scene arrays are inconsistent";
+		return false;
+	}
+	const std::string tmp = path + ".tmp";
+	std::FILE *f = std::fopen(tmp.c_str(), "wb");
+	if (!f) {
+		err = "cannot open " + tmp + " for writing";
+		return false;
+	}
+	// header with a placeholder checksum, patched after the payload
+	uint32_t hdr[4] = {kCacheVersion, (uint32_t)n, (uint32_t)sc.textures.size(), 0u};
+	uint64_t sum = 0;
+	bool ok = std::fwrite(kCacheMagic, 1, 8, f) == 8 && std::fwrite(hdr, 4, 4, f) == 4 && std::fwrite(&sum, 8, 1, f) == 1;
+	CacheHash hs;
+	ok = ok && put(f, hs, sc.verts) && put(f, hs, sc.albedo) && put(f, hs, sc.emission) && put(f, hs, sc.normals) &&
+	     put(f, hs, sc.colour) && put(f, hs, sc.material) && put(f, hs, sc.texture_id) && put(f, hs, sc.texcoords);
+	for (const Texture &t : sc.textures) {
+		if (!ok)
+			break;
+		int32_t wh[2] = {t.width, t.height};
+		hs.add(wh, sizeof wh);
+		ok = std::fwrite(wh, 4, 2, f) == 2 && put(f, hs, t.argb);
+	}
+	sum = hs.finish();
+	ok = ok && std::fseek(f, 24, SEEK_SET) == 0 && std::fwrite(&sum, 8, 1, f) == 1;
+	ok = (std::fclose(f) == 0) && ok;
+	if (!ok || std::rename(tmp.c_str(), path.c_str()) != 0) {
+		std::remove(tmp.c_str());
+		err = "write failed: " + path;
+		return false;
+	}
+	return true;
+}
+
+bool load_scene(const std::string &path, HostScene &sc, std::string &err) {
+	std::FILE *f = std::fopen(path.c_str(), "rb");
+	if (!f) {
+		err = "cannot open " + path;
+		return false;
+	}
+	struct Closer {
+		std::FILE *f;
+		~Closer() { std::fclose(f); }
+	} closer{f};
+	char magic[8];
+	uint32_t hdr[4];
+	uint64_t sum = 0;
+	if (std::fread(magic, 1, 8, f) != 8 || std::memcmp(magic, kCacheMagic, 8) != 0) {
+		err = path + " is not a scene cache file";
+		return false;
+	}
+	if (std::fread(hdr, 4, 4, f) != 4 || std::fread(&sum, 8, 1, f) != 1) {
+		err = path + ": truncated header";
+		return false;
+	}
+	if (hdr[0] != kCacheVersion) {
+		err = path + ": unsupported scene cache version " + std::to_string(hdr[0]);
+		return false;
+	}
+	const size_t n = hdr[1], nt = hdr[2];
+	if (n == 0 || n > (1u << 30) || nt > (1u << 20)) {
+		err = path + ": implausible triangle or texture count";
+		return false;
+	}
+	HostScene h;
+	h.n = (int)n;
+	CacheHash hs;
+	bool ok = get(f, hs, h.verts, 9 * n) && get(f, hs, h.albedo, 3 * n) && get(f, hs, h.emission, 3 * n) &&
+	          get(f, hs, h.normals, 3 * n) && get(f, hs, h.colour, 3 * n) && get(f, hs, h.material, n) &&
+	          get(f, hs, h.texture_id, n) && get(f, hs, h.texcoords, 6 * n);
+	h.textures.resize(nt);
+	for (size_t k = 0; ok && k < nt; k++) {
+		int32_t wh[2];
+		ok = std::fread(wh, 4, 2, f) == 2;
+		if (!ok)
+			break;
+		hs.add(wh, sizeof wh);
+		if (wh[0] < 0 || wh[1] < 0 || (int64_t)wh[0] * wh[1] > (int64_t)1 << 28) {
+			err = path + ": implausible texture size";
+			return false;
+		}
+		h.textures[k].width = wh[0];
+		h.textures[k].height = wh[1];
+		ok = get(f, hs, h.textures[k].argb, (size_t)wh[0] * (size_t)wh[1]);
+	}
+	if (!ok) {
+		err = path + ": truncated";
+		return false;
+	}
+	if (std::fgetc(f) != EOF) {
+		err = path + ": trailing bytes after the scene";
+		return false;
+	}
+	if (hs.finish() != sum) {
+		err = path + ": checksum mismatch (corrupt scene cache)";
+		return false;
+	}
+	for (int id : h.texture_id)
+		if (id < -1 || id >= (int)nt) {
+			err = path + ": texture id out of range";
+			return false;
+		}
+	sc = std::move(h);
+	return true;
+}
+
 } // namespace rn

@@ -32,4 +32,8 @@ bool load_ppm(const std::string &path, Texture &tex, std::string &err);
 void face_normal(const float *tri9, float *out3);
 bool write_png(const std::string &path, int w, int h, const unsigned char *rgba);
 
+// Binary scene cache (raw arrays + checksum); see scene_io.cpp for the layout.
+bool save_scene(const std::string &path, const HostScene &sc, std::string &err);
+bool load_scene(const std::string &path, HostScene &sc, std::string &err);
+
 } // namespace rn
