@@ -224,6 +224,28 @@ int rn_radix_sort_pairs(uint32_t *keys, uint32_t *values, int n, int bits);
 /* Philox4x32-10 on the GPU: out[4i..4i+3] = philox(ctr[4i..4i+3], key[2i..2i+1]). */
 int rn_philox4x32_10(int n, const uint32_t *ctr, const uint32_t *key, uint32_t *out);
 
+/* Progressive rendering: an accumulator holds the float64 running sum of
+ * the physical estimator over frames. rn_accum_render adds params->spp
+ * samples per pixel, numbered from the samples already held, so k frames of
+ * s samples render the same paths as one frame of k*s (params->width and
+ * height must equal the accumulator's). rn_accum_resolve_rgba8 writes the
+ * running mean tone-mapped and packed in one GPU pass: c = mean * exposure,
+ * c = c / (c + 1), c = c^(1/gamma) * 255, byte = min(255, (int)c), RGBA
+ * order, alpha 255 -- the steps of the reference's toneMap
+ * (src/Renderer.cpp:1357-1368) on the HDR mean, its OpenCL epilogue
+ * (gpu/pathtracer.cl:122-132) for exposure = gamma = 1, and rn_render's
+ * byte order (src/capi.cpp:69-77). rn_accum_mean writes the mean as FP32
+ * (3 floats per pixel). Functions return 1 / non-NULL on success, else 0 /
+ * NULL with rn_last_error() set; rn_accum_samples returns the count. */
+typedef struct rn_accum rn_accum;
+rn_accum *rn_accum_create(int width, int height);
+void rn_accum_free(rn_accum *acc);
+int rn_accum_reset(rn_accum *acc);
+int rn_accum_render(rn_accum *acc, const rn_scene *scene, const rn_camera *camera, const rn_render_params *params);
+int rn_accum_samples(const rn_accum *acc);
+int rn_accum_resolve_rgba8(const rn_accum *acc, float exposure, float gamma, unsigned char *rgba);
+int rn_accum_mean(const rn_accum *acc, float *hdr_rgb);
+
 /* Name of the GPU the library runs on (e.g. "gfx950"), or "" if none. */
 const char *rn_device_name(void);
 

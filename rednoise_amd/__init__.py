@@ -173,6 +173,10 @@ def _configure(lib: C.CDLL) -> None:
         "rn_render": ([vp, i, i, i, f, i, vp], i), "rn_render_seeded": ([vp, i, i, f, i, C.c_uint32, C.c_uint32, vp], i), "rn_save_png": ([cp, i, i, vp], i), "rn_version": ([], cp),
         "rn_last_error": ([], cp), "rn_device_name": ([], cp), "rn_release_workspace": ([], i),
         "rn_scene_save": ([vp, cp], i), "rn_scene_load": ([cp], vp),
+        "rn_accum_create": ([i, i], vp), "rn_accum_free": ([vp], None), "rn_accum_reset": ([vp], i),
+        "rn_accum_render": ([vp, vp, C.POINTER(Camera), C.POINTER(RenderParams)], i),
+        "rn_accum_samples": ([vp], i), "rn_accum_resolve_rgba8": ([vp, f, f, vp], i),
+        "rn_accum_mean": ([vp, vp], i),
         "rn_scene_from_arrays": ([vp, i, vp, vp], vp), "rn_scene_set_materials": ([vp, vp, vp], i),
         "rn_scene_get_triangles": ([vp, vp, vp, vp], i), "rn_scene_get_compat": ([vp, vp, vp, vp, vp, vp], i),
         "rn_scene_get_texture": ([vp, i, vp, vp, vp], i),
@@ -210,6 +214,56 @@ def version() -> str:
 
 def device_name() -> str:
     return _load_library().rn_device_name().decode()
+
+
+class Accumulator:
+    """Progressive renderer state (``rn_accum_*``): a float64 running sum on
+    the GPU. ``render`` adds ``params.spp`` samples per pixel continuing the
+    sample numbering, ``resolve`` tone-maps and packs RGBA8 in one GPU pass,
+    ``mean`` returns the HDR running mean."""
+
+    def __init__(self, width: int, height: int):
+        lib = _load_library()
+        self._lib = lib
+        self._h = lib.rn_accum_create(int(width), int(height))
+        if not self._h:
+            raise _err("rn_accum_create")
+        self.width, self.height = int(width), int(height)
+
+    @property
+    def samples(self) -> int:
+        return int(self._lib.rn_accum_samples(self._h))
+
+    def reset(self) -> None:
+        if self._lib.rn_accum_reset(self._h) != 1:
+            raise _err("rn_accum_reset")
+
+    def render(self, scene: "Scene", camera: "Camera", params: "RenderParams") -> None:
+        if self._lib.rn_accum_render(self._h, scene.handle, C.byref(camera), C.byref(params)) != 1:
+            raise _err("rn_accum_render")
+
+    def resolve(self, exposure: float = 1.0, gamma: float = 1.0) -> np.ndarray:
+        out = np.empty(self.width * self.height * 4, np.uint8)
+        if self._lib.rn_accum_resolve_rgba8(self._h, C.c_float(exposure), C.c_float(gamma), _ptr(out)) != 1:
+            raise _err("rn_accum_resolve_rgba8")
+        return out.reshape(self.height, self.width, 4)
+
+    def mean(self) -> np.ndarray:
+        out = np.empty(self.width * self.height * 3, np.float32)
+        if self._lib.rn_accum_mean(self._h, _ptr(out)) != 1:
+            raise _err("rn_accum_mean")
+        return out.reshape(self.height, self.width, 3)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.rn_accum_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def release_workspace() -> None:

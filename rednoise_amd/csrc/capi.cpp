@@ -317,6 +317,7 @@ DevParams make_params(const rn_render_params &p, const rn_scene *sc, uint32_t sc
 	d.seed = p.seed;
 	d.scene_id = scene_id;
 	d.ray_eps = p.ray_eps > 0.0f ? p.ray_eps : 1e-4f * sc->radius;
+	d.sample_base = 0;
 	return d;
 }
 
@@ -610,9 +611,18 @@ int rn_save_png(const char *path, int width, int height, const unsigned char *rg
 // Device-pointer entry points run on the caller's stream (NULL = the legacy
 // default stream, which orders them with torch's default stream); host-pointer
 // entry points run on the library's own stream.
+struct rn_accum {
+	int width = 0, height = 0;
+	long long samples = 0;
+	DevBuf sum; // 3 doubles per pixel
+	std::mutex mu;
+};
+
+// With acc, the frame (spp samples numbered from acc->samples) is rendered
+// into the library's staging buffer and added to the accumulator's sum.
 static int render_common(const rn_scene *scene, const rn_camera *camera, const rn_render_params *params,
                          const float *d_albedo, const float *d_emission, float *d_out, hipStream_t stream,
-                         bool caller_stream, float *h_out) {
+                         bool caller_stream, float *h_out, rn_accum *acc = nullptr) {
 	if (!scene || !camera || !params)
 		throw Error("NULL scene, camera or params");
 	auto t0 = std::chrono::steady_clock::now();
@@ -621,6 +631,13 @@ static int render_common(const rn_scene *scene, const rn_camera *camera, const r
 	rn_scene *sc = const_cast<rn_scene *>(scene);
 	std::lock_guard<std::mutex> sl(sc->mu);
 	DevParams p = make_params(*params, sc, 0);
+	if (acc) {
+		if (p.width != acc->width || p.height != acc->height)
+			throw Error("params width/height differ from the accumulator's");
+		if (acc->samples + p.spp > 0x7FFFFFFFll)
+			throw Error("accumulator sample count would exceed 2^31 - 1");
+		p.sample_base = (uint32_t)acc->samples;
+	}
 	DevCamera cam = make_camera(*camera, p.width, p.height);
 	hipStream_t s = caller_stream ? stream : c.stream;
 	bool had = sc->bvh_valid && sc->bvh.width == (params->bvh_width > 0 ? params->bvh_width : 8);
@@ -645,12 +662,16 @@ static int render_common(const rn_scene *scene, const rn_camera *camera, const r
 		launch_mega(ds, cam, p, out, cnt, (params->flags & RN_FLAG_COUNT_TRAVERSAL) != 0, s);
 	}
 	RN_HIP(hipEventRecord(c.ev1, s));
+	if (acc)
+		launch_accum_add(acc->sum.as<double>(), out, npx * 3, (double)p.spp, s);
 	DevCounters hcs;
 	download<DevCounters>(&hcs, cnt, 1, s);
 	const CounterSet hc = hcs.total();
 	if (h_out)
 		download<float>(h_out, out, npx * 3, s);
 	RN_HIP(hipStreamSynchronize(s));
+	if (acc)
+		acc->samples += p.spp;
 	float ms = 0;
 	RN_HIP(hipEventElapsedTime(&ms, c.ev0, c.ev1));
 	std::memset(&t_stats, 0, sizeof t_stats);
@@ -692,6 +713,95 @@ int rn_render_ex_device(const rn_scene *scene, const rn_camera *camera, const rn
 		throw Error("d_hdr_rgb is NULL");
 	return render_common(scene, camera, params, d_albedo, d_emission, d_hdr_rgb, static_cast<hipStream_t>(stream),
 	                     true, nullptr);
+	RN_CATCH(0)
+}
+
+rn_accum *rn_accum_create(int width, int height) {
+	RN_TRY
+	t_error.clear();
+	if (width <= 0 || height <= 0 || (long long)width * height > (1ll << 31) / 4) {
+		set_error("width and height must be positive and the image at most 2^29 pixels");
+		return nullptr;
+	}
+	Context &c = ctx();
+	std::unique_ptr<rn_accum> a(new rn_accum());
+	a->width = width;
+	a->height = height;
+	std::lock_guard<std::mutex> lk(c.mu);
+	a->sum.reserve((size_t)width * height * 3 * sizeof(double));
+	fill_zero(a->sum.ptr, (size_t)width * height * 3 * sizeof(double), c.stream);
+	RN_HIP(hipStreamSynchronize(c.stream));
+	return a.release();
+	RN_CATCH(nullptr)
+}
+
+void rn_accum_free(rn_accum *acc) { delete acc; }
+
+int rn_accum_reset(rn_accum *acc) {
+	RN_TRY
+	t_error.clear();
+	if (!acc)
+		throw Error("accumulator is NULL");
+	// lock order everywhere: accumulator, then library context
+	std::lock_guard<std::mutex> al(acc->mu);
+	Context &c = ctx();
+	std::lock_guard<std::mutex> lk(c.mu);
+	fill_zero(acc->sum.ptr, (size_t)acc->width * acc->height * 3 * sizeof(double), c.stream);
+	RN_HIP(hipStreamSynchronize(c.stream));
+	acc->samples = 0;
+	return 1;
+	RN_CATCH(0)
+}
+
+int rn_accum_render(rn_accum *acc, const rn_scene *scene, const rn_camera *camera, const rn_render_params *params) {
+	RN_TRY
+	t_error.clear();
+	if (!acc)
+		throw Error("accumulator is NULL");
+	std::lock_guard<std::mutex> al(acc->mu);
+	return render_common(scene, camera, params, nullptr, nullptr, nullptr, nullptr, false, nullptr, acc);
+	RN_CATCH(0)
+}
+
+int rn_accum_samples(const rn_accum *acc) { return acc ? (int)acc->samples : 0; }
+
+int rn_accum_resolve_rgba8(const rn_accum *acc, float exposure, float gamma, unsigned char *rgba) {
+	RN_TRY
+	t_error.clear();
+	if (!acc || !rgba)
+		throw Error("accumulator and rgba must be non-NULL");
+	if (!(exposure > 0.0f) || !(gamma > 0.0f))
+		throw Error("exposure and gamma must be positive");
+	std::lock_guard<std::mutex> al(const_cast<rn_accum *>(acc)->mu);
+	if (acc->samples <= 0)
+		throw Error("the accumulator holds no samples");
+	Context &c = ctx();
+	std::lock_guard<std::mutex> lk(c.mu);
+	const size_t npx = (size_t)acc->width * acc->height;
+	unsigned char *d = static_cast<unsigned char *>(c.o4.reserve(npx * 4));
+	launch_tonemap_rgba8(acc->sum.as<double>(), npx, 1.0 / (double)acc->samples, exposure, gamma, d, c.stream);
+	download<unsigned char>(rgba, d, npx * 4, c.stream);
+	RN_HIP(hipStreamSynchronize(c.stream));
+	return 1;
+	RN_CATCH(0)
+}
+
+int rn_accum_mean(const rn_accum *acc, float *hdr_rgb) {
+	RN_TRY
+	t_error.clear();
+	if (!acc || !hdr_rgb)
+		throw Error("accumulator and hdr_rgb must be non-NULL");
+	std::lock_guard<std::mutex> al(const_cast<rn_accum *>(acc)->mu);
+	if (acc->samples <= 0)
+		throw Error("the accumulator holds no samples");
+	Context &c = ctx();
+	std::lock_guard<std::mutex> lk(c.mu);
+	const size_t n = (size_t)acc->width * acc->height * 3;
+	float *d = static_cast<float *>(c.o3.reserve(n * 4));
+	launch_mean(acc->sum.as<double>(), n, 1.0 / (double)acc->samples, d, c.stream);
+	download<float>(hdr_rgb, d, n, c.stream);
+	RN_HIP(hipStreamSynchronize(c.stream));
+	return 1;
 	RN_CATCH(0)
 }
 

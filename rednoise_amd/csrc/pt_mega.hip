@@ -133,7 +133,7 @@ __global__ __launch_bounds__(kMBlock) void k_mega(DevScene sc, DevCamera cam, De
 	if (valid) {
 		float3 sum = f3(0.f, 0.f, 0.f);
 		for (int s = 0; s < p.spp; s++)
-			sum = sum + trace_sample<LDS, SPILL, false, COUNT>(sc, cam, p, x, y, (uint32_t)s, stk, threadIdx.x, ncl,
+			sum = sum + trace_sample<LDS, SPILL, false, COUNT>(sc, cam, p, x, y, p.sample_base + (uint32_t)s, stk, threadIdx.x, ncl,
 			                                                   nsh, sum, nullptr, cc, cs);
 		float inv = 1.0f / (float)p.spp;
 		size_t o = 3 * ((size_t)y * p.width + x);
@@ -176,7 +176,7 @@ __global__ __launch_bounds__(kMBlock) void k_adjoint(DevScene sc, DevCamera cam,
 		unsigned long long ncl = 0, nsh = 0;
 		if (w.x != 0.f || w.y != 0.f || w.z != 0.f)
 			for (int s = 0; s < p.spp; s++)
-				trace_sample<LDS, SPILL, true>(sc, cam, p, x, y, (uint32_t)s, stk, threadIdx.x, ncl, nsh, w, &gs);
+				trace_sample<LDS, SPILL, true>(sc, cam, p, x, y, p.sample_base + (uint32_t)s, stk, threadIdx.x, ncl, nsh, w, &gs);
 	}
 	__syncthreads();
 	for (int i = threadIdx.x; i < 3 * n_hot; i += kMBlock) {

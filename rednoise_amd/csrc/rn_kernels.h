@@ -63,4 +63,11 @@ void launch_adjoint(const DevScene &sc, const DevCamera &cam, const DevParams &p
 void launch_compat_render(const DevScene &sc, const CompatDev &cd, int W, int H, float cam_z, int spp, uint32_t key0,
                           uint32_t key1, unsigned char *rgba, hipStream_t s);
 
+// Progressive accumulation (float64 sum += weight * mean) and the fused
+// tone-map + RGBA8 pack of sum * inv_count; the mean as FP32.
+void launch_accum_add(double *sum, const float *mean, size_t n, double weight, hipStream_t s);
+void launch_tonemap_rgba8(const double *sum, size_t npx, double inv_count, float exposure, float gamma,
+                          unsigned char *rgba, hipStream_t s);
+void launch_mean(const double *sum, size_t n, double inv_count, float *out, hipStream_t s);
+
 } // namespace rn

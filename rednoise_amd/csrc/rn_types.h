@@ -45,6 +45,7 @@ struct DevParams {
 	float rr_prob;
 	uint32_t seed, scene_id;
 	float ray_eps;
+	uint32_t sample_base; // first sample index: a progressive render continues where it stopped
 };
 
 // Device view of one scene (pointers into HBM).

@@ -146,7 +146,7 @@ __global__ __launch_bounds__(256) void k_generate(WQ q, int cur) {
 		int si = find_scene(q.scenes, q.nscenes, g);
 		const WScene &ws = q.scenes[si];
 		unsigned long long local = g - ws.first;
-		uint32_t smp = (uint32_t)(local / (unsigned long long)ws.npix_pad);
+		uint32_t smp = ws.p.sample_base + (uint32_t)(local / (unsigned long long)ws.npix_pad);
 		uint32_t pp = (uint32_t)(local % (unsigned long long)ws.npix_pad);
 		uint32_t tile = pp >> 6, t = pp & 63u;
 		int x = (int)(tile % (uint32_t)ws.tiles_x) * kPixTile + (int)(t & 7u);
