@@ -247,6 +247,30 @@ def terrain_bench(rn, lib, torch):
     return res
 
 
+def progressive_bench(rn, sc, cam, W, H, depth, rr, eps):
+    """Progressive 1080p rendering of the config-1 scene: 8 frames of 8 spp
+    into one accumulator (wavefront), each followed by the fused tone-map +
+    RGBA8 pack and its download."""
+    acc = rn.Accumulator(W, H)
+    p = rn.RenderParams.make(W, H, 8, depth, rr, seed=0, ray_eps=eps, integrator=rn.INTEGRATOR_WAVEFRONT)
+    acc.render(sc, cam, p)  # warm-up
+    acc.resolve()
+    acc.reset()
+    frames, t_render, t_resolve = 8, 0.0, 0.0
+    for _ in range(frames):
+        t0 = time.perf_counter()
+        acc.render(sc, cam, p)
+        t1 = time.perf_counter()
+        acc.resolve(1.0, 2.2)
+        t_resolve += time.perf_counter() - t1
+        t_render += t1 - t0
+    res = {"workload": f"config1 scene, {W}x{H}, {frames} frames x 8 spp, depth {depth}, tone-map exposure 1 gamma 2.2",
+           "ms_per_frame": round(1e3 * t_render / frames, 2), "resolve_ms": round(1e3 * t_resolve / frames, 3),
+           "samples": acc.samples}
+    acc.close()
+    return res
+
+
 def scene_cache_bench(rn, sd):
     """The config-1 scene written as an OBJ, then loaded with
     rn_scene_load_obj (the reference's entry point) and with rn_scene_load
@@ -456,6 +480,11 @@ def main():
             log(f"batch: {result['batch']}")
         except Exception as e:
             result["batch"] = {"error": str(e)}
+        try:
+            result["progressive"] = progressive_bench(rn, sc, cam, W, H, depth, rr, eps)
+            log(f"progressive: {result['progressive']}")
+        except Exception as e:
+            result["progressive"] = {"error": str(e)}
         try:
             result["scene_cache"] = scene_cache_bench(rn, sd)
             log(f"scene_cache: {result['scene_cache']}")
