@@ -63,6 +63,9 @@ int rn_scene_get_compat(const rn_scene *scene, float *normals, int *colour, int 
 
 /* Texture `id` of an OBJ scene: writes width/height, and if argb != NULL the
  * width*height 0xAARRGGBB pixels (framework/TextureMap.cpp:4-31). */
+/* Smoothed vertex normals, 9 floats per triangle (loadOBJ's vertexNormals,
+ * src/ObjLoader.cpp:178-205; a scene from arrays gets its face normals). */
+int rn_scene_get_vertex_normals(const rn_scene *scene, float *vnormals);
 int rn_scene_get_texture(const rn_scene *scene, int id, int *width, int *height, uint32_t *argb);
 
 /* ------------------------------------------------------------ rendering -- */

@@ -27,6 +27,19 @@ static std::vector<ModelTriangle> from_verts(const float *v, int n) {
 
 extern "C" {
 
+// The smoothed vertex normals of loadOBJ (src/ObjLoader.cpp:178-205), 9 per
+// triangle; returns the triangle count.
+int ref_load_obj_vnormals(const char *path, float scale, float *vn) {
+	std::vector<ModelTriangle> m = loadOBJ(path, scale);
+	int n = static_cast<int>(m.size());
+	if (vn)
+		for (int i = 0; i < n; i++)
+			for (int k = 0; k < 3; k++)
+				for (int a = 0; a < 3; a++)
+					vn[9 * i + 3 * k + a] = m[static_cast<size_t>(i)].vertexNormals[k][a];
+	return n;
+}
+
 // loadOBJ (src/ObjLoader.cpp:102-214) flattened: returns the triangle count;
 // fills the arrays when non-NULL (verts 9n, normals 3n, colour 3n, material n,
 // has_texture n, texcoords 6n).

@@ -173,6 +173,7 @@ def _configure(lib: C.CDLL) -> None:
         "rn_render": ([vp, i, i, i, f, i, vp], i), "rn_render_seeded": ([vp, i, i, f, i, C.c_uint32, C.c_uint32, vp], i), "rn_save_png": ([cp, i, i, vp], i), "rn_version": ([], cp),
         "rn_last_error": ([], cp), "rn_device_name": ([], cp), "rn_release_workspace": ([], i),
         "rn_scene_save": ([vp, cp], i), "rn_scene_load": ([cp], vp),
+        "rn_scene_get_vertex_normals": ([vp, vp], i),
         "rn_accum_create": ([i, i], vp), "rn_accum_free": ([vp], None), "rn_accum_reset": ([vp], i),
         "rn_accum_render": ([vp, vp, C.POINTER(Camera), C.POINTER(RenderParams)], i),
         "rn_accum_samples": ([vp], i), "rn_accum_resolve_rgba8": ([vp, f, f, vp], i),
@@ -466,6 +467,13 @@ class Scene:
         _load_library().rn_scene_get_compat(self._handle, _ptr(nm), _ptr(col), _ptr(mat), _ptr(tid), _ptr(tc))
         return {"normals": nm.reshape(-1, 3), "colour": col.reshape(-1, 3), "material": mat, "texture_id": tid,
                 "texcoords": tc.reshape(-1, 6)}
+
+    def vertex_normals(self) -> np.ndarray:
+        """(n, 3, 3) smoothed vertex normals (face normals for array scenes)."""
+        out = np.empty(9 * self._n, np.float32)
+        if _load_library().rn_scene_get_vertex_normals(self._handle, _ptr(out)) != 1:
+            raise _err("rn_scene_get_vertex_normals")
+        return out.reshape(-1, 3, 3)
 
     def texture(self, tex_id: int) -> np.ndarray:
         w, h = C.c_int(), C.c_int()

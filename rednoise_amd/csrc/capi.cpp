@@ -111,7 +111,7 @@ struct rn_scene {
 	float radius = 1.0f;
 	int n_hot = 0;
 	DevBuf d_verts, d_albedo, d_emission, d_light_tri, d_light_cdf, d_light_geom, d_hot_slot, d_hot_prim;
-	DevBuf d_cnorm, d_ccol, d_ctc, d_texels, d_texinfo;
+	DevBuf d_cnorm, d_vnorm, d_ccol, d_ctc, d_texels, d_texinfo;
 	Bvh bvh;
 
 	// Device copies of geometry and materials, built on first use.
@@ -217,6 +217,7 @@ struct rn_scene {
 		upload(d_cnorm, cn.data(), cn.size(), s);
 		upload(d_ccol, cc.data(), cc.size(), s);
 		upload(d_ctc, h.texcoords.data(), h.texcoords.size(), s);
+		upload(d_vnorm, h.vnormals.data(), h.vnormals.size(), s);
 		std::vector<uint32_t> texels;
 		std::vector<int4> info;
 		for (const Texture &t : h.textures) {
@@ -260,6 +261,7 @@ struct rn_scene {
 	CompatDev compat() const {
 		CompatDev c;
 		c.cnorm = d_cnorm.as<float4>();
+		c.vnorm = d_vnorm.as<float>();
 		c.ccol = d_ccol.as<float4>();
 		c.ctc = d_ctc.as<float>();
 		c.texels = d_texels.as<uint32_t>();
@@ -415,6 +417,7 @@ rn_scene *rn_scene_from_arrays(const float *verts, int n, const float *albedo, c
 		for (int a = 0; a < 3; a++)
 			h.colour[3 * i + a] = (int)std::lround(std::min(1.0f, std::max(0.0f, albedo[3 * i + a])) * 255.0f);
 	}
+	flat_vertex_normals(h);
 	return s.release();
 	RN_CATCH(nullptr)
 }
@@ -520,6 +523,15 @@ int rn_scene_get_compat(const rn_scene *scene, float *normals, int *colour, int 
 	return 1;
 }
 
+int rn_scene_get_vertex_normals(const rn_scene *scene, float *vnormals) {
+	if (!scene || !vnormals) {
+		set_error("scene and vnormals must be non-NULL");
+		return 0;
+	}
+	std::memcpy(vnormals, scene->h.vnormals.data(), scene->h.vnormals.size() * 4);
+	return 1;
+}
+
 int rn_scene_get_texture(const rn_scene *scene, int id, int *width, int *height, uint32_t *argb) {
 	if (!scene || id < 0 || id >= (int)scene->h.textures.size()) {
 		set_error("no such texture");
@@ -568,6 +580,37 @@ static int render_compat(const rn_scene *scene, int width, int height, float cam
 	return 1;
 }
 
+// RN_RAYTRACED: the reference's Whitted renderer (deterministic; samples
+// is ignored, as in src/capi.cpp:59-61).
+static int render_whitted(const rn_scene *scene, int width, int height, float cam_z, unsigned char *rgba) {
+	if ((long long)width * height > (1ll << 29))
+		throw Error("rn_render: image too large");
+	auto t0 = std::chrono::steady_clock::now();
+	Context &c = ctx();
+	std::lock_guard<std::mutex> lk(c.mu);
+	rn_scene *sc = const_cast<rn_scene *>(scene);
+	std::lock_guard<std::mutex> sl(sc->mu);
+	bool had = sc->bvh_valid;
+	sc->ensure_bvh(8, c.stream);
+	unsigned char *d_rgba = static_cast<unsigned char *>(c.out.reserve((size_t)width * height * 4));
+	RN_HIP(hipEventRecord(c.ev0, c.stream));
+	launch_whitted_render(sc->dev(), sc->compat(), width, height, cam_z, d_rgba, c.stream);
+	RN_HIP(hipEventRecord(c.ev1, c.stream));
+	download<unsigned char>(rgba, d_rgba, (size_t)width * height * 4, c.stream);
+	RN_HIP(hipStreamSynchronize(c.stream));
+	float ms = 0;
+	RN_HIP(hipEventElapsedTime(&ms, c.ev0, c.ev1));
+	std::memset(&t_stats, 0, sizeof t_stats);
+	t_stats.build_ms = had ? 0.0 : sc->bvh.build_ms;
+	t_stats.render_ms = ms;
+	t_stats.total_ms = ms_since(t0);
+	t_stats.trace_kernel_ms = ms;
+	t_stats.trace_kernel_launches = 1;
+	t_stats.bvh_nodes = sc->bvh.n_nodes;
+	t_stats.bvh_depth = sc->bvh.depth;
+	return 1;
+}
+
 int rn_render(const rn_scene *scene, rn_render_mode mode, int width, int height, float cam_z, int samples,
               unsigned char *rgba) {
 	RN_TRY
@@ -576,8 +619,10 @@ int rn_render(const rn_scene *scene, rn_render_mode mode, int width, int height,
 		set_error("rn_render: NULL scene/buffer or non-positive size");
 		return 0;
 	}
+	if (mode == RN_RAYTRACED)
+		return render_whitted(scene, width, height, cam_z, rgba);
 	if (mode != RN_PATHTRACED) {
-		set_error("rn_render: only RN_PATHTRACED is implemented by this library");
+		set_error("rn_render: only RN_PATHTRACED and RN_RAYTRACED are implemented by this library");
 		return 0;
 	}
 	return render_compat(scene, width, height, cam_z, samples, kCompatKey0, kCompatKey1, rgba);

@@ -138,23 +138,26 @@ RN_DEV float3 reflectv(float3 i, float3 n) { return i - n * (dot3(n, i) * 2.0f);
 #define RN_LPOS f3(0.0f, 0.95f, 0.0f)
 constexpr float kLRadius = 0.15f, kLIntensity = 40.0f, kLKq = 12.566370f;
 
+// src/Renderer.cpp:483-497 (the default light's 9-point disk) in glm's
+// operation order: normalize, cross, length and the division to / dist.
 template <int LDS, int SPILL>
 RN_DEV float visibility(const DevScene &sc, float3 point, int ignore, uint64_t *lds, int lane) {
+#pragma clang fp contract(off)
 	const float ks[9][2] = {{0, 0},       {1, 0},        {-1, 0},       {0, 1},        {0, -1},
 	                        {0.7f, 0.7f}, {-0.7f, 0.7f}, {0.7f, -0.7f}, {-0.7f, -0.7f}};
 	float3 lp = RN_LPOS;
-	float3 axis = normalize3(point - lp);
+	float3 axis = gnormalize(point - lp);
 	float3 up = fabsf(axis.y) < 0.99f ? f3(0, 1, 0) : f3(1, 0, 0);
-	float3 tangent = normalize3(cross3(up, axis));
-	float3 bitangent = cross3(axis, tangent);
+	float3 tangent = gnormalize(gcross(up, axis));
+	float3 bitangent = gcross(axis, tangent);
 	int un = 0;
 #pragma unroll 1
 	for (int i = 0; i < 9; i++) {
 		float3 spos = lp + (tangent * ks[i][0] + bitangent * ks[i][1]) * kLRadius;
 		float3 to = spos - point;
-		float dist = sqrtf(dot3(to, to));
-		Hit h = traverse<1, 1, LDS, SPILL>(sc.nodes, sc.tris, point, to * (1.0f / dist), dist, ignore, lds, kCBlock,
-		                                   lane);
+		float dist = sqrtf(gdot(to, to));
+		float3 dir = f3(to.x / dist, to.y / dist, to.z / dist);
+		Hit h = traverse<1, 1, LDS, SPILL>(sc.nodes, sc.tris, point, dir, dist, ignore, lds, kCBlock, lane);
 		if (h.prim == 0)
 			un++;
 	}
@@ -309,7 +312,243 @@ __global__ __launch_bounds__(kCBlock) void k_compat(DevScene sc, CompatDev cd, i
 	reinterpret_cast<uchar4 *>(rgba)[pix] = px;
 }
 
+// ------------------------------------------------------------ Whitted --
+// rn_render(RN_RAYTRACED): renderRaytraced (src/Renderer.cpp:909-948) with
+// the arguments rn_render passes (src/capi.cpp:59-61): Phong shading, the
+// default soft point light, maxDepth 4, no fog. traceRay's recursion
+// (src/Renderer.cpp:735-772) is a depth-first walk over an explicit stack of
+// weighted rays: every branch (mirror, the two glass rays weighted by
+// Fresnel, the three dispersive channels, metal tinted by its colour) is
+// linear in the colours returned below it, so the pixel is the sum of
+// weight x leaf colour. The leaf colours keep glm's FP32 operation order.
+
+struct WhRay {
+	float3 o, d, w;
+	int depth;
+};
+constexpr int kWhStack = 24; // depth 4 x (6 dispersive children - 1) + 1
+
+RN_DEV float3 gmix(float3 a, float3 b, float t) {
+#pragma clang fp contract(off)
+	return a * (1.0f - t) + b * t;
+}
+
+RN_DEV float3 wh_environment(float3 d) { // src/Renderer.cpp:555-563
+#pragma clang fp contract(off)
+	float t = 0.5f * (d.y + 1.0f);
+	return gmix(f3(200.0f, 210.0f, 235.0f), f3(70.0f, 120.0f, 205.0f), t);
+}
+
+RN_DEV float3 greflect(float3 i, float3 n) {
+#pragma clang fp contract(off)
+	float k = gdot(n, i);
+	return i - n * k * 2.0f;
+}
+
+// src/Renderer.cpp:574-581
+RN_DEV float3 bump_normal(float3 n, float3 p) {
+#pragma clang fp contract(off)
+	float3 q = p * 6.0f;
+	const float e = 0.15f;
+	float3 g = f3(perlin(q.x + e, q.y, q.z) - perlin(q.x - e, q.y, q.z),
+	              perlin(q.x, q.y + e, q.z) - perlin(q.x, q.y - e, q.z),
+	              perlin(q.x, q.y, q.z + e) - perlin(q.x, q.y, q.z - e));
+	return gnormalize(n + g * 2.0f);
+}
+
+// src/Renderer.cpp:618-639
+RN_DEV float3 tangent_normal(const CompatDev &cd, const float4 *tris, uint32_t leaf, int prim, int tex, float u,
+                             float v, float3 gn) {
+#pragma clang fp contract(off)
+	float3 v0 = xyz(tris[3 * leaf]), v1 = xyz(tris[3 * leaf + 1]), v2 = xyz(tris[3 * leaf + 2]);
+	float3 e1 = v1 - v0, e2 = v2 - v0;
+	const float *tp = cd.ctc + 6 * (size_t)prim;
+	float d1x = tp[2] - tp[0], d1y = tp[3] - tp[1], d2x = tp[4] - tp[0], d2y = tp[5] - tp[1];
+	float det = d1x * d2y - d2x * d1y;
+	if (fabsf(det) < 1e-8f)
+		return gn;
+	float f = 1.0f / det;
+	float3 T = gnormalize((e1 * d2y - e2 * d1y) * f);
+	float3 B = gnormalize((e1 * -d2x + e2 * d1x) * f);
+	int4 ti = cd.tex_info[tex];
+	float e = 1.0f / (float)ti.y;
+	auto lum = [&](float uu, float vv) {
+		float3 c = texel_rgb(cd, tex, uu, vv);
+		return (c.x + c.y + c.z) / 765.0f;
+	};
+	float hu = lum(u + e, v) - lum(u - e, v);
+	float hv = lum(u, v + e) - lum(u, v - e);
+	float3 tn = gnormalize(f3(-hu * 4.0f, -hv * 4.0f, 1.0f));
+	return gnormalize(T * tn.x + B * tn.y + gn * tn.z);
+}
+
+// shadeSurface (src/Renderer.cpp:503-534) for the default light.
+template <int LDS, int SPILL>
+RN_DEV float3 shade_surface(const DevScene &sc, float3 point, float3 n, float3 view, int ignore, float3 base,
+                            uint64_t *lds, int lane) {
+#pragma clang fp contract(off)
+	const float ambient = 0.2f;
+	float3 to = RN_LPOS - point;
+	float dist = sqrtf(gdot(to, to));
+	float3 ldir = f3(to.x / dist, to.y / dist, to.z / dist);
+	float att = kLIntensity / (0.0f + 0.0f * dist + kLKq * dist * dist);
+	float vis = visibility<LDS, SPILL>(sc, point, ignore, lds, lane);
+	float inc = fmaxf(0.0f, gdot(n, ldir));
+	float bright = vis * 1.0f * att * inc;
+	float3 diff = f3(0.0f, 0.0f, 0.0f) + f3(1.0f, 1.0f, 1.0f) * bright;
+	float spec = 0.0f;
+	if (bright > 0.0f) {
+		float3 rd = n * (2.0f * gdot(n, ldir)) - ldir;
+		spec += vis * 1.0f * powf(fmaxf(0.0f, gdot(rd, view)), 64.0f);
+	}
+	float3 lit = diff + f3(ambient, ambient, ambient);
+	lit = f3(fminf(1.0f, lit.x), fminf(1.0f, lit.y), fminf(1.0f, lit.z));
+	float s255 = 255.0f * fminf(1.0f, spec);
+	return f3(base.x * lit.x + s255, base.y * lit.y + s255, base.z * lit.z + s255);
+}
+
+// shadeDiffuse with Phong normals (src/Renderer.cpp:662-699).
+template <int LDS, int SPILL>
+RN_DEV float3 shade_diffuse(const DevScene &sc, const CompatDev &cd, const Hit &h, float3 point, float3 d,
+                            uint64_t *lds, int lane) {
+#pragma clang fp contract(off)
+	const int prim = h.prim;
+	float w0 = 1.0f - h.u - h.v, w1 = h.u, w2 = h.v;
+	float3 view = d * -1.0f;
+	float3 base = base_colour(cd, sc.tris, h.leaf, prim, h.u, h.v, point, view);
+	const float *vn = cd.vnorm + 9 * (size_t)prim;
+	float3 n0 = f3(vn[0], vn[1], vn[2]), n1 = f3(vn[3], vn[4], vn[5]), n2 = f3(vn[6], vn[7], vn[8]);
+	float3 n = gnormalize(n0 * w0 + n1 * w1 + n2 * w2);
+	n = face_viewer(n, d);
+	int mat = __float_as_int(cd.cnorm[prim].w);
+	if (mat == CM_BUMP)
+		n = bump_normal(n, point);
+	int tex = __float_as_int(cd.ccol[prim].w);
+	if (mat == CM_NORMALMAP && tex >= 0) {
+		float tu, tv;
+		compat_uv(cd, prim, h.u, h.v, tu, tv);
+		n = tangent_normal(cd, sc.tris, h.leaf, prim, tex, tu, tv, n);
+	}
+	return shade_surface<LDS, SPILL>(sc, point, n, view, prim, base, lds, lane);
+}
+
+// glassTrace (src/Renderer.cpp:710-731): push the reflected and refracted
+// rays weighted by Fresnel (the reflected ray alone on total internal
+// reflection).
+RN_DEV void glass_push(WhRay *st, int &sp, float3 point, float3 d, float3 N, float3 w, int depth, float ior) {
+#pragma clang fp contract(off)
+	float3 n = N;
+	float cosi = gdot(d, n);
+	float etai = 1.0f, etat = ior;
+	if (cosi < 0.0f) {
+		cosi = -cosi;
+	} else {
+		etai = ior;
+		etat = 1.0f;
+		n = n * -1.0f;
+	}
+	float3 refl = greflect(d, n);
+	float eta = etai / etat;
+	float dv = gdot(n, d);
+	float k = 1.0f - eta * eta * (1.0f - dv * dv);
+	float3 refr = k >= 0.0f ? d * eta - n * (eta * dv + sqrtf(k)) : f3(0.0f, 0.0f, 0.0f);
+	if (sqrtf(gdot(refr, refr)) < 1e-6f) {
+		if (sp < kWhStack)
+			st[sp++] = WhRay{point + refl * 1e-4f, refl, w, depth - 1};
+		return;
+	}
+	float r0 = (etai - etat) / (etai + etat);
+	r0 *= r0;
+	float fres = r0 + (1.0f - r0) * powf(1.0f - cosi, 5.0f);
+	if (sp < kWhStack)
+		st[sp++] = WhRay{point + refr * 1e-4f, refr, w * (1.0f - fres), depth - 1};
+	if (sp < kWhStack)
+		st[sp++] = WhRay{point + refl * 1e-4f, refl, w * fres, depth - 1};
+}
+
+template <int LDS, int SPILL>
+RN_DEV float3 whitted_pixel(const DevScene &sc, const CompatDev &cd, float3 o, float3 d, uint64_t *lds, int lane) {
+#pragma clang fp contract(off)
+	WhRay st[kWhStack];
+	int sp = 0;
+	st[sp++] = WhRay{o, d, f3(1.0f, 1.0f, 1.0f), 4};
+	float3 col = f3(0.0f, 0.0f, 0.0f);
+	while (sp > 0) {
+		const WhRay r = st[--sp];
+		Hit h = traverse<0, 1, LDS, SPILL>(sc.nodes, sc.tris, r.o, r.d, INFINITY, -1, lds, kCBlock, lane);
+		if (h.prim < 0) {
+			col = col + r.w * wh_environment(r.d);
+			continue;
+		}
+		const int ti = h.prim;
+		const float3 point = r.o + r.d * h.t;
+		const float4 nm = cd.cnorm[ti];
+		const float3 N = xyz(nm);
+		const int mat = __float_as_int(nm.w);
+		const float3 kd = xyz(cd.ccol[ti]);
+		if (mat == CM_EMISSIVE) {
+			col = col + r.w * (kd * 1.3f);
+			continue;
+		}
+		if (r.depth > 0 && mat == CM_MIRROR) {
+			float3 n = face_viewer(N, r.d);
+			float3 rr = greflect(r.d, n);
+			if (sp < kWhStack)
+				st[sp++] = WhRay{point + rr * 1e-4f, rr, r.w, r.depth - 1};
+			continue;
+		}
+		if (r.depth > 0 && mat == CM_GLASS) {
+			glass_push(st, sp, point, r.d, N, r.w, r.depth, 1.5f);
+			continue;
+		}
+		if (r.depth > 0 && mat == CM_DISPERSIVE) {
+			glass_push(st, sp, point, r.d, N, r.w * f3(1.0f, 0.0f, 0.0f), r.depth, 1.50f);
+			glass_push(st, sp, point, r.d, N, r.w * f3(0.0f, 1.0f, 0.0f), r.depth, 1.53f);
+			glass_push(st, sp, point, r.d, N, r.w * f3(0.0f, 0.0f, 1.0f), r.depth, 1.56f);
+			continue;
+		}
+		if (r.depth > 0 && mat == CM_METAL) {
+			float3 n = face_viewer(N, r.d);
+			float3 rr = greflect(r.d, n);
+			if (sp < kWhStack)
+				st[sp++] = WhRay{point + rr * 1e-4f, rr, r.w * (kd * (1.0f / 255.0f)), r.depth - 1};
+			continue;
+		}
+		col = col + r.w * shade_diffuse<LDS, SPILL>(sc, cd, h, point, r.d, lds, lane);
+	}
+	return col;
+}
+
+template <int LDS, int SPILL>
+__global__ __launch_bounds__(kCBlock) void k_whitted(DevScene sc, CompatDev cd, int W, int H, float cam_z,
+                                                     unsigned char *rgba) {
+	__shared__ uint64_t stk[LDS * kCBlock];
+	uint32_t b = xcd_block(blockIdx.x, gridDim.x);
+	int tiles_x = (W + kTileW - 1) / kTileW;
+	int x = (int)(b % tiles_x) * kTileW + (threadIdx.x % kTileW);
+	int y = (int)(b / tiles_x) * kTileH + (threadIdx.x / kTileW);
+	if (x >= W || y >= H)
+		return;
+	float3 o, d;
+	// renderRaytraced's pixel position x - W/2 (no half-pixel offset)
+	compat_primary_ray(x, y, W, H, cam_z, -0.5f, -0.5f, o, d);
+	float3 c = whitted_pixel<LDS, SPILL>(sc, cd, o, d, stk, threadIdx.x);
+	int rr = min(255, (int)c.x), gg = min(255, (int)c.y), bb = min(255, (int)c.z);
+	reinterpret_cast<uchar4 *>(rgba)[y * W + x] =
+	    make_uchar4((unsigned char)(rr & 0xFF), (unsigned char)(gg & 0xFF), (unsigned char)(bb & 0xFF), 255);
+}
+
 } // namespace
+
+void launch_whitted_render(const DevScene &sc, const CompatDev &cd, int W, int H, float cam_z, unsigned char *rgba,
+                           hipStream_t s) {
+	unsigned g = (unsigned)(((W + kTileW - 1) / kTileW) * ((H + kTileH - 1) / kTileH));
+	if (sc.depth - 1 > kLdsDepth)
+		k_whitted<kLdsDepth, kSpillDepth><<<g, kCBlock, 0, s>>>(sc, cd, W, H, cam_z, rgba);
+	else
+		k_whitted<kLdsDepth, 0><<<g, kCBlock, 0, s>>>(sc, cd, W, H, cam_z, rgba);
+	RN_HIP(hipGetLastError());
+}
 
 void launch_compat_render(const DevScene &sc, const CompatDev &cd, int W, int H, float cam_z, int spp, uint32_t key0,
                           uint32_t key1, unsigned char *rgba, hipStream_t s) {

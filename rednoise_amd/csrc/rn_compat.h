@@ -8,6 +8,7 @@
 // Per-triangle attributes of the reference path, original triangle order.
 struct CompatDev {
 	const float4 *cnorm;     // face normal xyz, w = material id (int bits)
+	const float *vnorm;      // smoothed vertex normals, 9 per triangle
 	const float4 *ccol;      // Kd colour ints as floats 0..255, w = texture id (int bits, -1 none)
 	const float *ctc;        // texture coordinates, 6 per triangle
 	const uint32_t *texels;  // every texture's 0xAARRGGBB pixels, concatenated

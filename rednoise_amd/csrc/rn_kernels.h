@@ -59,6 +59,10 @@ void launch_adjoint(const DevScene &sc, const DevCamera &cam, const DevParams &p
                     const int *hot_slot, const int *hot_prim, int n_hot, float *g_albedo, float *g_emission,
                     hipStream_t s);
 
+// rn_render(RN_RAYTRACED): the reference's Whitted renderer, RGBA8 output.
+void launch_whitted_render(const DevScene &sc, const CompatDev &cd, int W, int H, float cam_z, unsigned char *rgba,
+                           hipStream_t s);
+
 // Reference estimator (rn_render RN_PATHTRACED): RGBA8 output.
 void launch_compat_render(const DevScene &sc, const CompatDev &cd, int W, int H, float cam_z, int spp, uint32_t key0,
                           uint32_t key1, unsigned char *rgba, hipStream_t s);

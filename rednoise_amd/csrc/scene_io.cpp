@@ -154,7 +154,8 @@ bool load_ppm(const std::string &path, Texture &tex, std::string &err) {
 }
 
 // src/ObjLoader.cpp:102-214. Face normals use glm's cross/length order; the
-// smoothed vertex normals are not built (the path tracer uses face normals).
+// smoothed vertex normals (used by the Whitted renderer) follow
+// src/ObjLoader.cpp:178-205 with glm's dot/length/normalize order.
 bool load_obj(const std::string &path, float scale, HostScene &sc, std::string &err) {
 	std::ifstream file(path);
 	if (!file.is_open()) {
@@ -276,6 +277,46 @@ bool load_obj(const std::string &path, float scale, HostScene &sc, std::string &
 			sc.texture_id[i] = f.texture;
 		}
 	}
+	// Smoothed vertex normals: per vertex, the normals of the adjacent faces
+	// (in face order) within ~60 degrees of this face's (dot > 0.5), summed
+	// and normalised; the face normal when the sum has zero length.
+	{
+#pragma STDC FP_CONTRACT OFF
+		std::vector<std::vector<int>> vfaces((size_t)nv);
+		for (size_t i = 0; i < n; i++) {
+			vfaces[(size_t)faces[i].a].push_back((int)i);
+			vfaces[(size_t)faces[i].b].push_back((int)i);
+			vfaces[(size_t)faces[i].c].push_back((int)i);
+		}
+		sc.vnormals.resize(9 * n);
+		auto dot = [](const float *a, const float *b) {
+			float x = a[0] * b[0], y = a[1] * b[1], z = a[2] * b[2];
+			return x + y + z;
+		};
+		for (size_t i = 0; i < n; i++) {
+			const float *fi = &sc.normals[3 * i];
+			int idx[3] = {faces[i].a, faces[i].b, faces[i].c};
+			for (int k = 0; k < 3; k++) {
+				float sum[3] = {0.0f, 0.0f, 0.0f};
+				for (int g : vfaces[(size_t)idx[k]]) {
+					const float *fg = &sc.normals[3 * (size_t)g];
+					if (dot(fg, fi) > 0.5f)
+						for (int a = 0; a < 3; a++)
+							sum[a] += fg[a];
+				}
+				float *out = &sc.vnormals[9 * i + 3 * k];
+				float len = std::sqrt(dot(sum, sum));
+				if (len > 0.0f) {
+					float inv = 1.0f / std::sqrt(dot(sum, sum));
+					for (int a = 0; a < 3; a++)
+						out[a] = sum[a] * inv;
+				} else {
+					for (int a = 0; a < 3; a++)
+						out[a] = fi[a];
+				}
+			}
+		}
+	}
 	// Physical-estimator view of an OBJ scene: albedo = Kd int / 255, the
 	// Emissive material radiates its colour * 4 / 255 (src/Renderer.cpp:829-833).
 	sc.albedo.resize(3 * n);
@@ -287,6 +328,15 @@ bool load_obj(const std::string &path, float scale, HostScene &sc, std::string &
 				sc.emission[3 * i + a] = (float)sc.colour[3 * i + a] / 255.0f * 4.0f;
 		}
 	return true;
+}
+
+void flat_vertex_normals(HostScene &sc) {
+	const size_t n = (size_t)sc.n;
+	sc.vnormals.resize(9 * n);
+	for (size_t i = 0; i < n; i++)
+		for (int k = 0; k < 3; k++)
+			for (int a = 0; a < 3; a++)
+				sc.vnormals[9 * i + 3 * k + a] = sc.normals[3 * i + a];
 }
 
 // Face normal with glm's operation order: cross(v1-v0, v2-v0) / length, or
@@ -409,7 +459,8 @@ bool write_png(const std::string &path, int w, int h, const unsigned char *rgba)
 //   u32          reserved (0)
 //   u64          checksum of everything after this header
 //   f32 verts[9n], f32 albedo[3n], f32 emission[3n], f32 normals[3n],
-//   i32 colour[3n], i32 material[n], i32 texture_id[n], f32 texcoords[6n],
+//   f32 vertex normals[9n], i32 colour[3n], i32 material[n],
+//   i32 texture_id[n], f32 texcoords[6n],
 //   then per texture: i32 width, i32 height, u32 argb[width * height].
 // The checksum folds the payload as 64-bit words (the tail zero-padded):
 // h = rotl(h ^ w, 29) * 0x9E3779B97F4A7C15, starting from the byte count.
@@ -488,7 +539,8 @@ template <class T> bool get(std::FILE *f, CacheHash &hs, std::vector<T> &v, size
 bool save_scene(const std::string &path, const HostScene &sc, std::string &err) {
 	const size_t n = (size_t)sc.n;
 	if (sc.verts.size() != 9 * n || sc.albedo.size() != 3 * n || sc.emission.size() != 3 * n ||
-	    sc.normals.size() != 3 * n || sc.colour.size() != 3 * n || sc.material.size() != n ||
+	    sc.normals.size() != 3 * n || sc.vnormals.size() != 9 * n || sc.colour.size() != 3 * n ||
+	    sc.material.size() != n ||
 	    sc.texture_id.size() != n || sc.texcoords.size() != 6 * n) {
 		err = "scene arrays are inconsistent";
 		return false;
@@ -504,7 +556,7 @@ bool save_scene(const std::string &path, const HostScene &sc, std::string &err) 
 	uint64_t sum = 0;
 	bool ok = std::fwrite(kCacheMagic, 1, 8, f) == 8 && std::fwrite(hdr, 4, 4, f) == 4 && std::fwrite(&sum, 8, 1, f) == 1;
 	CacheHash hs;
-	ok = ok && put(f, hs, sc.verts) && put(f, hs, sc.albedo) && put(f, hs, sc.emission) && put(f, hs, sc.normals) &&
+	ok = ok && put(f, hs, sc.verts) && put(f, hs, sc.albedo) && put(f, hs, sc.emission) && put(f, hs, sc.normals) && put(f, hs, sc.vnormals) &&
 	     put(f, hs, sc.colour) && put(f, hs, sc.material) && put(f, hs, sc.texture_id) && put(f, hs, sc.texcoords);
 	for (const Texture &t : sc.textures) {
 		if (!ok)
@@ -558,7 +610,7 @@ bool load_scene(const std::string &path, HostScene &sc, std::string &err) {
 	h.n = (int)n;
 	CacheHash hs;
 	bool ok = get(f, hs, h.verts, 9 * n) && get(f, hs, h.albedo, 3 * n) && get(f, hs, h.emission, 3 * n) &&
-	          get(f, hs, h.normals, 3 * n) && get(f, hs, h.colour, 3 * n) && get(f, hs, h.material, n) &&
+	          get(f, hs, h.normals, 3 * n) && get(f, hs, h.vnormals, 9 * n) && get(f, hs, h.colour, 3 * n) && get(f, hs, h.material, n) &&
 	          get(f, hs, h.texture_id, n) && get(f, hs, h.texcoords, 6 * n);
 	h.textures.resize(nt);
 	for (size_t k = 0; ok && k < nt; k++) {

@@ -12,6 +12,9 @@ Outputs (small .npz fixtures next to this script):
   modeR_cornell.npz    rn_render(RN_PATHTRACED) RGBA8 images of the Cornell
                        asset (reference estimator, per-pixel mt19937)
   noise.npz            fractalNoise(p, 4) at seeded points
+With --whitted (only these files):
+  whitted_<asset>.npz  rn_render(RN_RAYTRACED) RGBA8 images (160x120, 96x64,
+                       cam_z 4) and loadOBJ's smoothed vertex normals
 """
 from __future__ import annotations
 
@@ -66,6 +69,31 @@ def brute(ref, verts, o, d):
     v = np.empty(nr, np.float32)
     ref.ref_brute_closest(verts, n, nr, o, d, prim, t, u, v)
     return prim, t, u, v
+
+
+WHITTED = {"cornell": ("cornell-box.obj", 0.35, 4.0), "quad": ("textured-quad.obj", 0.35, 4.0),
+           "sphere": ("sphere.obj", 1.0, 4.0)}
+
+
+def whitted(ref) -> None:
+    """rn_render(RN_RAYTRACED) images of the compiled reference and loadOBJ's
+    smoothed vertex normals, per asset: whitted_<asset>.npz."""
+    ref.ref_load_obj_vnormals.argtypes = [C.c_char_p, C.c_float, C.c_void_p]
+    ref.ref_load_obj_vnormals.restype = C.c_int
+    for key, (name, scale, cz) in WHITTED.items():
+        path = os.path.join(ASSETS, name).encode()
+        n = ref.ref_load_obj_vnormals(path, scale, None)
+        vn = np.empty(9 * n, np.float32)
+        ref.ref_load_obj_vnormals(path, scale, vn.ctypes.data)
+        sc = ref.rn_scene_load_obj(path, C.c_float(scale))
+        out = {"vnormals": vn, "scale": np.float32(scale), "cam_z": np.float32(cz)}
+        for (W, H) in [(160, 120), (96, 64)]:
+            buf = np.empty(W * H * 4, np.uint8)
+            assert ref.rn_render(sc, 2, W, H, C.c_float(cz), 1, buf) == 1
+            out[f"rgba_{W}x{H}"] = buf.reshape(H, W, 4)
+        ref.rn_scene_free(sc)
+        np.savez_compressed(os.path.join(HERE, f"whitted_{key}.npz"), **out)
+        print("whitted", key, {k: v.shape for k, v in out.items()})
 
 
 def main() -> None:
@@ -124,4 +152,10 @@ def main() -> None:
 
 
 if __name__ == "__main__":
-    main()
+    if "--whitted" in sys.argv:
+        r = O.ref()
+        if r is None:
+            raise SystemExit("build the reference first: make -C oracle ref")
+        whitted(r)
+    else:
+        main()

@@ -53,7 +53,7 @@ def test_error_contract_without_device(rn):
     buf = np.zeros(16, np.uint8)
     assert lib.rn_render(None, 3, 2, 2, C.c_float(4.0), 4, buf.ctypes.data) == 0
     sc = rn.Scene.load_obj(os.path.join(ASSETS, "cornell-box.obj"))
-    for args in [(3, 0, 2, 4), (3, 2, -1, 4), (7, 2, 2, 4), (2, 2, 2, 4), (3, 2, 2, 0)]:
+    for args in [(3, 0, 2, 4), (3, 2, -1, 4), (7, 2, 2, 4), (0, 2, 2, 4), (1, 2, 2, 4), (3, 2, 2, 0)]:
         mode, w, h, spp = args
         assert lib.rn_render(sc.handle, mode, w, h, C.c_float(4.0), spp, buf.ctypes.data) == 0
         assert lib.rn_last_error()

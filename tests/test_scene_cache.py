@@ -60,8 +60,8 @@ def test_round_trip_arrays_and_layout(rn, tmp_path):
     off = 0
     arrays = {}
     for name, dt, k in [("verts", "<f4", 9), ("albedo", "<f4", 3), ("emission", "<f4", 3), ("normals", "<f4", 3),
-                        ("colour", "<i4", 3), ("material", "<i4", 1), ("texture_id", "<i4", 1),
-                        ("texcoords", "<f4", 6)]:
+                        ("vnormals", "<f4", 9), ("colour", "<i4", 3), ("material", "<i4", 1),
+                        ("texture_id", "<i4", 1), ("texcoords", "<f4", 6)]:
         arrays[name] = np.frombuffer(body, dt, k * n, off)
         off += 4 * k * n
     assert off == len(body)
@@ -69,6 +69,8 @@ def test_round_trip_arrays_and_layout(rn, tmp_path):
     assert np.array_equal(arrays["albedo"], al.reshape(-1))
     assert np.array_equal(arrays["emission"], em.reshape(-1))
     assert np.all(arrays["texture_id"] == -1)
+    # a scene from arrays has no shared vertices: its vertex normals are its face normals
+    assert np.array_equal(arrays["vnormals"].reshape(n, 3, 3), np.repeat(arrays["normals"].reshape(n, 1, 3), 3, 1))
     # checksum: fold 64-bit little-endian words (tail zero-padded), then the byte count
     mask = (1 << 64) - 1
     h = 0
@@ -103,3 +105,14 @@ def test_corrupt_truncated_foreign(rn, tmp_path):
         rn.Scene.load(tmp_path / "obj")
     with pytest.raises(FileNotFoundError):
         rn.Scene.load(tmp_path / "missing")
+
+
+@pytest.mark.parametrize("key,asset,scale", [("cornell", "cornell-box.obj", 0.35), ("quad", "textured-quad.obj", 0.35),
+                                             ("sphere", "sphere.obj", 1.0)])
+def test_vertex_normals_match_reference_loader(rn, key, asset, scale):
+    """Smoothed vertex normals bit-identical to loadOBJ's
+    (src/ObjLoader.cpp:178-205), golden from the compiled reference."""
+    g = np.load(os.path.join(ROOT, "tests", "golden", f"whitted_{key}.npz"))
+    sc = rn.Scene.load_obj(os.path.join(ASSETS, asset), scale)
+    ours = sc.vertex_normals().reshape(-1)
+    assert np.array_equal(ours.view(np.uint32), g["vnormals"].view(np.uint32))
