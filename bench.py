@@ -352,9 +352,27 @@ def reference_modeR(rn, torch):
     t0 = time.perf_counter()
     ours.render("pathtraced", W, H, 4.0, spp)
     t_gpu = time.perf_counter() - t0
-    return {"call": f"rn_render(RN_PATHTRACED) cornell-box.obj 320x240 spp={spp} cam_z=4",
-            "reference_cpu_s": round(t_ref, 4), "reference_threads": os.cpu_count(), "gpu_s": round(t_gpu, 5),
-            "speedup": round(t_ref / t_gpu, 1)}
+    res = {"call": f"rn_render(RN_PATHTRACED) cornell-box.obj 320x240 spp={spp} cam_z=4",
+           "reference_cpu_s": round(t_ref, 4), "reference_threads": os.cpu_count(), "gpu_s": round(t_gpu, 5),
+           "speedup": round(t_ref / t_gpu, 1)}
+    # the Whitted mode on the same asset, 1280x960
+    W2, H2 = 1280, 960
+    sc = ref.rn_scene_load_obj(asset.encode(), C.c_float(0.35))
+    buf2 = np.empty(W2 * H2 * 4, np.uint8)
+    t0 = time.perf_counter()
+    ref.rn_render(sc, 2, W2, H2, C.c_float(4.0), 1, buf2)
+    t_ref2 = time.perf_counter() - t0
+    ref.rn_scene_free(sc)
+    ours.render("raytraced", W2, H2, 4.0, 1)
+    t0 = time.perf_counter()
+    img = ours.render("raytraced", W2, H2, 4.0, 1).to_numpy()
+    t_gpu2 = time.perf_counter() - t0
+    same = float((img.reshape(-1, 4) == buf2.reshape(-1, 4)).all(axis=1).mean())
+    res["raytraced"] = {"call": f"rn_render(RN_RAYTRACED) cornell-box.obj {W2}x{H2} cam_z=4",
+                        "reference_cpu_s": round(t_ref2, 4), "gpu_s": round(t_gpu2, 5),
+                        "speedup": round(t_ref2 / t_gpu2, 1), "identical_pixels": round(same, 5)}
+    ours.close()
+    return res
 
 
 def main():

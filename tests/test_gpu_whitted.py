@@ -37,9 +37,8 @@ def test_whitted_matches_reference(rn, key, asset, scale, size):
     near = (d[~axis] <= 2).mean()
     assert np.all(img[..., 3] == 255)
     assert same > 0.995 and near > 0.999, (key, W, H, same, near, d[~axis].max())
-    # on the axis lines the two agree wherever the reference did not lose the hit
-    sky = ref[..., :3].sum(axis=2) > img[..., :3].sum(axis=2) + 150
-    assert (d[axis & ~sky] <= 2).mean() > 0.95
+    if key != "sphere":  # no node face lies on an axis plane of these rays: the lines agree too
+        assert (d[axis] <= 2).mean() > 0.99
 
 
 def test_whitted_deterministic_and_sized(rn):
