@@ -127,7 +127,7 @@ class Stats(C.Structure):
 
 class BvhInfo(C.Structure):
     _fields_ = [("nodes", C.c_int), ("depth", C.c_int), ("leaf_tris", C.c_int), ("width", C.c_int),
-                ("build_ms", C.c_double)]
+                ("build_ms", C.c_double), ("builder", C.c_int)]
 
 
 # ------------------------------------------------------------ the library --
@@ -190,6 +190,7 @@ def _configure(lib: C.CDLL) -> None:
         "rn_trace_rays": ([vp, i, vp, vp, vp, vp, i, i, vp, vp, vp, vp], i),
         "rn_trace_primary_compat": ([vp, i, i, f, vp, vp, vp, vp, vp], i),
         "rn_scene_build": ([vp, i, C.POINTER(BvhInfo)], i), "rn_scene_export_bvh": ([vp, vp, vp], i),
+        "rn_scene_set_builder": ([vp, i], i),
         "rn_lbvh_stages": ([vp, i, vp, vp, vp, vp], i), "rn_radix_sort_pairs": ([vp, vp, i, i], i),
         "rn_philox4x32_10": ([i, vp, vp, vp], i),
     }
@@ -482,6 +483,14 @@ class Scene:
         px = np.empty(w.value * h.value, np.uint32)
         _load_library().rn_scene_get_texture(self._handle, tex_id, C.byref(w), C.byref(h), _ptr(px))
         return px.reshape(h.value, w.value)
+
+    def set_builder(self, builder: str) -> None:
+        """Select the BVH builder: "ploc" (default) or "lbvh". Drops the built BVH."""
+        code = {"ploc": 0, "lbvh": 1}.get(builder)
+        if code is None:
+            raise ValueError(f"unknown BVH builder {builder!r}")
+        if _load_library().rn_scene_set_builder(self._handle, code) != 1:
+            raise _err("rn_scene_set_builder")
 
     def build(self, bvh_width: int = 8) -> dict:
         info = BvhInfo()

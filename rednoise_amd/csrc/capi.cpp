@@ -106,6 +106,7 @@ struct rn_scene {
 	std::mutex mu;
 	bool uploaded = false;
 	bool bvh_valid = false;
+	int builder = RN_DEFAULT_BUILDER; // RN_BVH_PLOC or RN_BVH_LBVH
 	int nl = 0;
 	float total_area = 0.0f;
 	float radius = 1.0f;
@@ -234,9 +235,9 @@ struct rn_scene {
 		ensure_device(s);
 		if (width <= 0)
 			width = 8;
-		if (bvh_valid && bvh.width == width)
+		if (bvh_valid && bvh.width == width && bvh.builder == builder)
 			return;
-		build_bvh(bvh, d_verts.as<float>(), h.n, width, s);
+		build_bvh(bvh, d_verts.as<float>(), h.n, width, s, builder);
 		if (bvh.depth > kMaxBvhDepth)
 			throw Error("BVH deeper than the traversal stack supports");
 		bvh_valid = true;
@@ -1079,7 +1080,23 @@ int rn_scene_build(rn_scene *scene, int bvh_width, rn_bvh_info *info) {
 		info->leaf_tris = scene->bvh.n_tris;
 		info->width = scene->bvh.width;
 		info->build_ms = scene->bvh.build_ms;
+		info->builder = scene->bvh.builder;
 	}
+	return 1;
+	RN_CATCH(0)
+}
+
+int rn_scene_set_builder(rn_scene *scene, int builder) {
+	RN_TRY
+	t_error.clear();
+	if (!scene)
+		throw Error("scene is NULL");
+	if (builder != RN_BVH_PLOC && builder != RN_BVH_LBVH)
+		throw Error("unknown BVH builder " + std::to_string(builder));
+	std::lock_guard<std::mutex> sl(scene->mu);
+	if (scene->builder != builder)
+		scene->bvh_valid = false;
+	scene->builder = builder;
 	return 1;
 	RN_CATCH(0)
 }

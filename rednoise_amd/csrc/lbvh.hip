@@ -260,7 +260,7 @@ __device__ __forceinline__ int delta(const uint32_t *k, int n, int i, int j) {
 
 // Internal node i of n-1; parent[] holds internal nodes at [0, n-1) and leaf
 // p at n-1+p; children >= 0 are internal, < 0 are leaves ~p.
-__global__ void k_karras(const uint32_t *k, int n, int *left, int *right, int *parent, int2 *range) {
+__global__ void k_karras(const uint32_t *k, int n, int *left, int *right, int *parent, int2 *range, uint32_t *count) {
 	int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= n - 1)
 		return;
@@ -290,6 +290,7 @@ __global__ void k_karras(const uint32_t *k, int n, int *left, int *right, int *p
 	parent[lc >= 0 ? lc : n - 1 + ~lc] = i;
 	parent[rc >= 0 ? rc : n - 1 + ~rc] = i;
 	range[i] = make_int2(lo, hi);
+	count[i] = (uint32_t)(hi - lo + 1);
 	if (i == 0)
 		parent[0] = -1;
 }
@@ -370,28 +371,144 @@ __global__ __launch_bounds__(kRefitChunk) void k_refit(const float *v, const uin
 	}
 }
 
+// ------------------------------------------------------------------ PLOC --
+// Parallel locally-ordered clustering (Meister and Bittner, "Parallel
+// Locally-Ordered Clustering for Bounding Volume Hierarchy Construction",
+// IEEE TVCG 2018): clusters start as the Morton-sorted triangles; each round
+// every cluster finds the neighbour within kPlocR positions whose merged box
+// has the least surface area, mutual nearest neighbours merge, and the
+// survivors are compacted in order. The result has the node layout of the
+// Karras tree (internal ids 0..n-2 with root 0, leaves ~position, boxes at
+// 2*id and 2*(n-1+position), triangle counts), so the collapse takes either.
+
+constexpr int kPlocR = 16;
+
+__global__ void k_leaf_boxes(const float *v, const uint32_t *order, int n, float4 *box, int *cref, float4 *cbox,
+                             uint32_t *ccnt) {
+	int p = blockIdx.x * blockDim.x + threadIdx.x;
+	if (p >= n)
+		return;
+	const float *t = v + 9 * (size_t)order[p];
+	float4 lo = make_float4(fminf(fminf(t[0], t[3]), t[6]), fminf(fminf(t[1], t[4]), t[7]),
+	                        fminf(fminf(t[2], t[5]), t[8]), 0.0f);
+	float4 hi = make_float4(fmaxf(fmaxf(t[0], t[3]), t[6]), fmaxf(fmaxf(t[1], t[4]), t[7]),
+	                        fmaxf(fmaxf(t[2], t[5]), t[8]), 0.0f);
+	size_t li = (size_t)(n - 1 + p);
+	box[2 * li] = lo;
+	box[2 * li + 1] = hi;
+	cref[p] = ~p;
+	cbox[2 * (size_t)p] = lo;
+	cbox[2 * (size_t)p + 1] = hi;
+	ccnt[p] = 1u;
+}
+
+__device__ __forceinline__ float merged_area(float4 alo, float4 ahi, float4 blo, float4 bhi) {
+	float dx = fmaxf(ahi.x, bhi.x) - fminf(alo.x, blo.x);
+	float dy = fmaxf(ahi.y, bhi.y) - fminf(alo.y, blo.y);
+	float dz = fmaxf(ahi.z, bhi.z) - fminf(alo.z, blo.z);
+	return dx * dy + dy * dz + dz * dx;
+}
+
+// Nearest neighbour of every cluster. Pairs are ordered by (merged area,
+// smaller index, larger index), a strict total order, so the least pair of
+// the round is always mutual and every round merges at least once.
+__global__ __launch_bounds__(kBlock) void k_ploc_nn(int m, const float4 *cbox, int *nn) {
+	__shared__ float4 slo[kBlock + 2 * kPlocR], shi[kBlock + 2 * kPlocR];
+	const int base = blockIdx.x * kBlock - kPlocR;
+	for (int k = threadIdx.x; k < kBlock + 2 * kPlocR; k += kBlock) {
+		int g = base + k;
+		if (g >= 0 && g < m) {
+			slo[k] = cbox[2 * (size_t)g];
+			shi[k] = cbox[2 * (size_t)g + 1];
+		}
+	}
+	__syncthreads();
+	const int i = blockIdx.x * kBlock + threadIdx.x;
+	if (i >= m)
+		return;
+	const int li = i - base;
+	const float4 alo = slo[li], ahi = shi[li];
+	float best = INFINITY;
+	int bj = -1;
+	const int j0 = max(0, i - kPlocR), j1 = min(m - 1, i + kPlocR);
+	for (int j = j0; j <= j1; j++) {
+		if (j == i)
+			continue;
+		float a = merged_area(alo, ahi, slo[j - base], shi[j - base]);
+		// candidates come in increasing j: on equal area the earlier one has
+		// the smaller (min, max) index pair, except across i, where j < i
+		// gives (j, i) and j > i gives (i, j): (j, i) < (i, j') as j < i
+		if (a < best) {
+			best = a;
+			bj = j;
+		}
+	}
+	nn[i] = bj;
+}
+
+__global__ void k_ploc_flags(int m, const int *nn, uint32_t *mflag, uint32_t *kflag) {
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= m)
+		return;
+	int j = nn[i];
+	bool mutual = j >= 0 && nn[j] == i;
+	mflag[i] = (mutual && i < j) ? 1u : 0u;
+	kflag[i] = (mutual && i > j) ? 0u : 1u;
+}
+
+__global__ void k_ploc_totals(int m, const uint32_t *mflag, const uint32_t *moff, const uint32_t *kflag,
+                              const uint32_t *koff, uint32_t *tot) {
+	tot[0] = moff[m - 1] + mflag[m - 1];
+	tot[1] = koff[m - 1] + kflag[m - 1];
+}
+
+__global__ void k_ploc_merge(int m, int id_top, const int *nn, const uint32_t *mflag, const uint32_t *moff,
+                             const uint32_t *kflag, const uint32_t *koff, const int *cref, const float4 *cbox,
+                             const uint32_t *ccnt, int *cref2, float4 *cbox2, uint32_t *ccnt2, int *left, int *right,
+                             uint32_t *count, float4 *box) {
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= m || !kflag[i])
+		return;
+	const uint32_t pos = koff[i];
+	float4 lo = cbox[2 * (size_t)i], hi = cbox[2 * (size_t)i + 1];
+	if (mflag[i]) {
+		const int j = nn[i];
+		const int id = id_top - (int)moff[i];
+		float4 blo = cbox[2 * (size_t)j], bhi = cbox[2 * (size_t)j + 1];
+		lo = make_float4(fminf(lo.x, blo.x), fminf(lo.y, blo.y), fminf(lo.z, blo.z), 0.0f);
+		hi = make_float4(fmaxf(hi.x, bhi.x), fmaxf(hi.y, bhi.y), fmaxf(hi.z, bhi.z), 0.0f);
+		left[id] = cref[i];
+		right[id] = cref[j];
+		const uint32_t c = ccnt[i] + ccnt[j];
+		count[id] = c;
+		box[2 * (size_t)id] = lo;
+		box[2 * (size_t)id + 1] = hi;
+		cref2[pos] = id;
+		ccnt2[pos] = c;
+	} else {
+		cref2[pos] = cref[i];
+		ccnt2[pos] = ccnt[i];
+	}
+	cbox2[2 * (size_t)pos] = lo;
+	cbox2[2 * (size_t)pos + 1] = hi;
+}
+
 // -------------------------------------------------------------- collapse --
 
 struct Entry {
 	int id; // >= 0 binary internal, < 0 leaf ~pos
 	float3 lo, hi;
-	int first, last;
+	int cnt; // triangles below
 };
 
-__device__ __forceinline__ Entry make_entry(int c, int n, const float4 *box, const int2 *range) {
+__device__ __forceinline__ Entry make_entry(int c, int n, const float4 *box, const uint32_t *count) {
 	Entry e;
 	e.id = c;
 	size_t bi = c >= 0 ? (size_t)c : (size_t)(n - 1 + ~c);
 	float4 lo = box[2 * bi], hi = box[2 * bi + 1];
 	e.lo = f3(lo.x, lo.y, lo.z);
 	e.hi = f3(hi.x, hi.y, hi.z);
-	if (c >= 0) {
-		int2 r = range[c];
-		e.first = r.x;
-		e.last = r.y;
-	} else {
-		e.first = e.last = ~c;
-	}
+	e.cnt = c >= 0 ? (int)count[c] : 1;
 	return e;
 }
 
@@ -411,7 +528,7 @@ __device__ __forceinline__ uint32_t scale_exp(float ext) {
 }
 
 __global__ void k_collapse(int k0, int k1, int width, int *task, const float4 *box, const int *left, const int *right,
-                           const int2 *range, const uint32_t *order, const float *verts, int n, Bvh8Node *nodes,
+                           const uint32_t *count, const uint32_t *order, const float *verts, int n, Bvh8Node *nodes,
                            float4 *tris, uint32_t *counters, int node_cap) {
 	int k = k0 + blockIdx.x * blockDim.x + threadIdx.x;
 	if (k >= k1)
@@ -420,11 +537,11 @@ __global__ void k_collapse(int k0, int k1, int width, int *task, const float4 *b
 	Entry e[8];
 	int m;
 	if (b < 0) {
-		e[0] = make_entry(b, n, box, range);
+		e[0] = make_entry(b, n, box, count);
 		m = 1;
 	} else {
-		e[0] = make_entry(left[b], n, box, range);
-		e[1] = make_entry(right[b], n, box, range);
+		e[0] = make_entry(left[b], n, box, count);
+		e[1] = make_entry(right[b], n, box, count);
 		m = 2;
 		// Open the largest-area internal entry until the node is full: large
 		// entries first (fewer levels), then small ones (tighter leaf boxes
@@ -444,8 +561,8 @@ __global__ void k_collapse(int k0, int k1, int width, int *task, const float4 *b
 			if (best < 0)
 				break;
 			int id = e[best].id;
-			e[best] = make_entry(left[id], n, box, range);
-			e[m++] = make_entry(right[id], n, box, range);
+			e[best] = make_entry(left[id], n, box, count);
+			e[m++] = make_entry(right[id], n, box, count);
 		}
 	}
 	float3 lo = e[0].lo, hi = e[0].hi;
@@ -483,7 +600,7 @@ __global__ void k_collapse(int k0, int k1, int width, int *task, const float4 *b
 	}
 	int n_int = 0, n_tri = 0;
 	for (int j = 0; j < m; j++) {
-		int cnt = e[j].last - e[j].first + 1;
+		int cnt = e[j].cnt;
 		if (cnt > kLeafMax)
 			n_int++;
 		else
@@ -530,15 +647,28 @@ __global__ void k_collapse(int k0, int k1, int width, int *task, const float4 *b
 			nd.qlo[a][s] = (uint8_t)fmin(fmax(qlo[a], 0.0), 255.0);
 			nd.qhi[a][s] = (uint8_t)fmin(fmax(qhi[a], 0.0), 255.0);
 		}
-		int cnt = q.last - q.first + 1;
+		int cnt = q.cnt;
 		if (cnt > kLeafMax) {
 			imask |= 1u << s;
 			task[child_base + ic] = q.id;
 			ic++;
 		} else {
 			nd.meta[s] = (uint8_t)((((1u << cnt) - 1u) << 5) | off);
-			for (int r = 0; r < cnt; r++) {
-				uint32_t prim = order[q.first + r];
+			// the subtree's (<= kLeafMax) sorted positions, left to right
+			int pos[kLeafMax], np = 0, stk[kLeafMax], sp = 0;
+			stk[sp++] = q.id;
+			while (sp > 0) {
+				int c = stk[--sp];
+				if (c < 0) {
+					if (np < kLeafMax)
+						pos[np++] = ~c;
+				} else if (sp + 2 <= kLeafMax) {
+					stk[sp++] = right[c];
+					stk[sp++] = left[c];
+				}
+			}
+			for (int r = 0; r < cnt && r < np; r++) {
+				uint32_t prim = order[pos[r]];
 				const float *t = verts + 9 * (size_t)prim;
 				size_t dst = 3 * (size_t)(tri_base + off + r);
 				tris[dst] = make_float4(t[0], t[1], t[2], __int_as_float((int)prim));
@@ -586,8 +716,50 @@ void fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s) {
 	k_fill_u32<<<(unsigned)std::min<size_t>(4096, grid_for(n)), kBlock, 0, s>>>(p, v, n);
 }
 
+// Binary tree for the collapse with PLOC over the Morton order (n >= 2).
+void ploc_tree(Bvh &b, const float *d_verts, int n, hipStream_t s) {
+	const uint32_t *order = b.vals.as<uint32_t>();
+	int *left = static_cast<int *>(b.left.reserve((size_t)(n - 1) * 4));
+	int *right = static_cast<int *>(b.right.reserve((size_t)(n - 1) * 4));
+	uint32_t *count = static_cast<uint32_t *>(b.count.reserve((size_t)(n - 1) * 4));
+	float4 *box = static_cast<float4 *>(b.boxes.reserve((size_t)(2 * n - 1) * 32));
+	int *cref = static_cast<int *>(b.ploc_ref.reserve((size_t)n * 2 * 4));
+	float4 *cbox = static_cast<float4 *>(b.ploc_box.reserve((size_t)n * 2 * 32));
+	uint32_t *ccnt = static_cast<uint32_t *>(b.ploc_cnt.reserve((size_t)n * 2 * 4));
+	int *nn = static_cast<int *>(b.ploc_nn.reserve((size_t)n * 4));
+	uint32_t *fl = static_cast<uint32_t *>(b.ploc_flags.reserve(((size_t)n * 4 + 4) * 4));
+	uint32_t *mflag = fl, *kflag = fl + n, *moff = fl + 2 * (size_t)n, *koff = fl + 3 * (size_t)n,
+	         *tot = fl + 4 * (size_t)n;
+	uint32_t *tmp = static_cast<uint32_t *>(b.scan_tmp.reserve(scan_tmp_words((size_t)n) * 4 + 16));
+	k_leaf_boxes<<<grid_for(n), kBlock, 0, s>>>(d_verts, order, n, box, cref, cbox, ccnt);
+	RN_HIP(hipGetLastError());
+	int m = n, cur = 0, id_top = n - 2;
+	uint32_t h[2];
+	while (m > 1) {
+		int *cr = cref + (size_t)cur * n, *cr2 = cref + (size_t)(cur ^ 1) * n;
+		float4 *cb = cbox + (size_t)cur * 2 * n, *cb2 = cbox + (size_t)(cur ^ 1) * 2 * n;
+		uint32_t *cc = ccnt + (size_t)cur * n, *cc2 = ccnt + (size_t)(cur ^ 1) * n;
+		k_ploc_nn<<<grid_for(m), kBlock, 0, s>>>(m, cb, nn);
+		k_ploc_flags<<<grid_for(m), kBlock, 0, s>>>(m, nn, mflag, kflag);
+		exclusive_scan(mflag, moff, (size_t)m, tmp, s);
+		exclusive_scan(kflag, koff, (size_t)m, tmp, s);
+		k_ploc_totals<<<1, 1, 0, s>>>(m, mflag, moff, kflag, koff, tot);
+		k_ploc_merge<<<grid_for(m), kBlock, 0, s>>>(m, id_top, nn, mflag, moff, kflag, koff, cr, cb, cc, cr2, cb2,
+		                                            cc2, left, right, count, box);
+		RN_HIP(hipMemcpyAsync(h, tot, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+		RN_HIP(hipStreamSynchronize(s));
+		if (h[0] == 0 || (int)h[1] != m - (int)h[0])
+			throw Error("PLOC: a clustering round merged nothing (internal error)");
+		id_top -= (int)h[0];
+		m = (int)h[1];
+		cur ^= 1;
+	}
+	if (id_top != -1)
+		throw Error("PLOC: node count mismatch (internal error)");
+}
+
 // Codes + sorted order + (n >= 2) binary tree and refitted boxes.
-void lbvh_front(Bvh &b, const float *d_verts, int n, hipStream_t s) {
+void lbvh_front(Bvh &b, const float *d_verts, int n, hipStream_t s, bool karras = true) {
 	uint32_t *bounds = static_cast<uint32_t *>(b.bounds.reserve(6 * kBoundStride * sizeof(uint32_t)));
 	uint32_t *keys = static_cast<uint32_t *>(b.keys.reserve((size_t)n * 4));
 	uint32_t *vals = static_cast<uint32_t *>(b.vals.reserve((size_t)n * 4));
@@ -605,8 +777,20 @@ void lbvh_front(Bvh &b, const float *d_verts, int n, hipStream_t s) {
 	int2 *range = static_cast<int2 *>(b.range.reserve((size_t)std::max(1, n - 1) * 8));
 	float4 *box = static_cast<float4 *>(b.boxes.reserve(nn * 32));
 	uint32_t *visit = static_cast<uint32_t *>(b.visit.reserve((size_t)std::max(1, n - 1) * 4));
+	if (!karras) {
+		if (n >= 2) {
+			ploc_tree(b, d_verts, n, s);
+		} else {
+			int *cref = static_cast<int *>(b.ploc_ref.reserve(8));
+			float4 *cbox = static_cast<float4 *>(b.ploc_box.reserve(64));
+			uint32_t *ccnt = static_cast<uint32_t *>(b.ploc_cnt.reserve(8));
+			k_leaf_boxes<<<1, kBlock, 0, s>>>(d_verts, vals, n, box, cref, cbox, ccnt);
+		}
+		return;
+	}
+	uint32_t *count = static_cast<uint32_t *>(b.count.reserve((size_t)std::max(1, n - 1) * 4));
 	if (n >= 2) {
-		k_karras<<<grid_for(n - 1), kBlock, 0, s>>>(keys, n, left, right, parent, range);
+		k_karras<<<grid_for(n - 1), kBlock, 0, s>>>(keys, n, left, right, parent, range, count);
 		fill_u32(visit, 0u, (size_t)(n - 1), s);
 	}
 	k_refit<<<grid_for(n, kRefitChunk), kRefitChunk, 0, s>>>(d_verts, vals, n, parent, left, right, range, box,
@@ -639,12 +823,15 @@ void radix_sort_pairs(uint32_t *d_keys, uint32_t *d_vals, uint32_t *d_keys_tmp, 
 	}
 }
 
-void build_bvh(Bvh &b, const float *d_verts, int n, int width, hipStream_t s) {
+void build_bvh(Bvh &b, const float *d_verts, int n, int width, hipStream_t s, int builder) {
 	auto t0 = std::chrono::steady_clock::now();
 	if (width != 4 && width != 8)
 		throw Error("bvh width must be 4 or 8");
+	if (builder != 0 && builder != 1)
+		throw Error("unknown BVH builder");
 	b.width = width;
 	b.n_tris = n;
+	b.builder = builder;
 	uint32_t *counters = static_cast<uint32_t *>(b.counters.reserve(4 * sizeof(uint32_t)));
 	if (n == 0) {
 		Bvh8Node *nodes = static_cast<Bvh8Node *>(b.nodes.reserve(sizeof(Bvh8Node)));
@@ -656,7 +843,7 @@ void build_bvh(Bvh &b, const float *d_verts, int n, int width, hipStream_t s) {
 		b.build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 		return;
 	}
-	lbvh_front(b, d_verts, n, s);
+	lbvh_front(b, d_verts, n, s, builder == 1);
 	float4 *tris = static_cast<float4 *>(b.tris.reserve((size_t)n * 48));
 	int cap = n / 2 + 64;
 	for (int attempt = 0; attempt < 3; attempt++) {
@@ -669,7 +856,7 @@ void build_bvh(Bvh &b, const float *d_verts, int n, int width, hipStream_t s) {
 		while (lo < hi) {
 			k_collapse<<<grid_for(hi - lo, 64), 64, 0, s>>>(
 			    (int)lo, (int)hi, width, task, b.boxes.as<float4>(), b.left.as<int>(), b.right.as<int>(),
-			    b.range.as<int2>(), b.vals.as<uint32_t>(), d_verts, n, nodes, tris, counters, cap);
+			    b.count.as<uint32_t>(), b.vals.as<uint32_t>(), d_verts, n, nodes, tris, counters, cap);
 			RN_HIP(hipMemcpyAsync(h, counters, 3 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
 			RN_HIP(hipStreamSynchronize(s));
 			depth++;

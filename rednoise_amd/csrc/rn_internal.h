@@ -44,12 +44,17 @@ struct Bvh {
 	double build_ms = 0.0;
 	// scratch
 	DevBuf bounds, keys, vals, keys2, vals2, hist, scan_tmp, left, right, parent, range, boxes, visit, task, counters;
+	DevBuf count, ploc_ref, ploc_box, ploc_cnt, ploc_nn, ploc_flags;
+	int builder = 0; // 0 = PLOC, 1 = Karras LBVH (the builder of the last build)
 };
 
 // Build the BVH of n triangles whose vertices (9n floats) are already on the
 // device. Stream-ordered; the call synchronises once per BVH level to learn
 // the level sizes.
-void build_bvh(Bvh &bvh, const float *d_verts, int n, int width, hipStream_t s);
+#ifndef RN_DEFAULT_BUILDER
+#define RN_DEFAULT_BUILDER 0
+#endif
+void build_bvh(Bvh &bvh, const float *d_verts, int n, int width, hipStream_t s, int builder = RN_DEFAULT_BUILDER);
 
 // Stages exposed for testing (device buffers in, host copies out).
 void lbvh_stages(const float *d_verts, int n, uint32_t *h_codes, int *h_order, int *h_left, int *h_right,

@@ -152,3 +152,17 @@ def test_scene_from_arrays_host_side(rn):
     assert np.allclose(sc.triangles()[1], 0.25)
     with pytest.raises(ValueError):
         sc.set_materials(albedo=np.zeros(5, np.float32))
+
+
+def test_builder_selection_host_side(rn):
+    from rednoise_amd import scenes
+    sd = scenes.cornell_box()
+    sc = rn.Scene.from_arrays(sd.verts, sd.albedo, sd.emission)
+    sc.set_builder("lbvh")
+    sc.set_builder("ploc")
+    with pytest.raises(ValueError):
+        sc.set_builder("sah")
+    lib = rn._load_library()
+    assert lib.rn_scene_set_builder(sc._handle, 7) == 0
+    assert b"builder" in lib.rn_last_error()
+    assert lib.rn_scene_set_builder(None, 0) == 0
