@@ -1,9 +1,9 @@
 #!/usr/bin/env python3
 """Benchmark of the path-trace hot path on one MI355X.
 
-Headline workload: a 1,048,576-triangle displaced sphere in
+Headline workload (config 1): a 1,048,576-triangle displaced sphere in
 the Cornell box, 1920x1080, 64 spp, max depth 8, Russian roulette from depth 3;
-every step rebuilds the LBVH/BVH8 on the GPU and renders the frame. `value` is
+every step rebuilds the BVH (PLOC tree, BVH8 collapse) on the GPU and renders the frame. `value` is
 the wavefront integrator's throughput in Mrays/s (closest-hit + shadow
 queries / wall time of the step, build included); the plain per-pixel
 megakernel is timed over the same number of steps beside it. Both run the same
@@ -209,7 +209,7 @@ def terrain_bench(rn, lib, torch):
     dt, st = timed_steps(rn, lib, sc, cam, p, out.data_ptr(), 1, 1, torch)
     rays = st[0]["rays_closest"] + st[0]["rays_shadow"]
     res = {"workload": f"config4: {sd.notes}, {W}x{H}, {spp} spp, depth {depth}, RR from depth {rr}, "
-                       "LBVH+BVH8 built on the GPU in the step, wavefront",
+                       "PLOC+BVH8 built on the GPU in the step, wavefront",
            "mrays_s": round(rays / dt / 1e6, 2), "seconds_per_step": round(dt, 3),
            "build_ms": round(st[0]["build_ms"], 2), "bvh": {"nodes": st[0]["bvh_nodes"], "depth": st[0]["bvh_depth"]},
            "rays": {"closest": st[0]["rays_closest"], "shadow": st[0]["rays_shadow"], "paths": st[0]["paths"]},
@@ -459,7 +459,7 @@ def main():
         "warmup": args.warmup, "ms_per_step": round(1e3 * dt_w / args.steps, 3), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded scene generator)",
         "config": {"workload": "config1: 1,048,576-tri displaced sphere + 12 Cornell tris, 1920x1080, 64 spp, "
-                               "depth 8, RR from depth 3, LBVH+BVH8 rebuilt on the GPU every step, wavefront "
+                               "depth 8, RR from depth 3, PLOC+BVH8 rebuilt on the GPU every step, wavefront "
                                "integrator", "triangles": sd.n, "width": W, "height": H, "spp": spp,
                    "max_depth": depth},
         "roofline": roofline,
