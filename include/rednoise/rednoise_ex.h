@@ -272,6 +272,55 @@ const char *rn_device_name(void);
  * their BVHs are untouched. Returns 1, or 0 with rn_last_error() set. */
 int rn_release_workspace(void);
 
+/* ---------------------------------------------- other estimators -- */
+
+/* Photon mapping, renderPhotonMapped (src/Renderer.cpp:1297-1346) split in
+ * its two passes, with the PhotonMap of src/Photon.cpp:5-46 on the GPU.
+ * The reference draws from one std::mt19937 stream across photons (seed 777)
+ * and one per image row for the final gather (seed 1000 + y); here photon i
+ * and pixel p own the Philox4x32-10 streams (i, 0, 'phot') and
+ * (p, 0, 'gath') under the key (key0, key1). Functions return 1 / non-NULL on
+ * success, else 0 / NULL with rn_last_error() set. */
+typedef struct rn_photon_map rn_photon_map;
+
+/* Pass 1 (src/Renderer.cpp:1310-1326): emit num_photons (0..2^26) from the
+ * default light (src/Renderer.cpp:1303-1308), trace them through mirrors,
+ * metal and glass (tracePhoton, :1177-1226) and build the map with cell side
+ * 0.25 (:1301). */
+rn_photon_map *rn_photon_map_trace(const rn_scene *scene, int num_photons, uint32_t key0, uint32_t key1);
+
+/* A map over given photons (3n floats each: position, power, direction of
+ * arrival), PhotonMap::photons + PhotonMap::build(cell) (src/Photon.cpp:10-19). */
+rn_photon_map *rn_photon_map_create(int n, const float *position, const float *power, const float *direction,
+                                    float cell);
+void rn_photon_map_free(rn_photon_map *map);
+/* Photons held (PhotonMap::photons.size()). */
+int rn_photon_map_count(const rn_photon_map *map);
+/* The photons in emission order (the reference's push order); NULL arrays
+ * are skipped. */
+int rn_photon_map_get(const rn_photon_map *map, float *position, float *power, float *direction);
+/* PhotonMap::gather (src/Photon.cpp:21-46) at m points with their normals
+ * (3m floats each): irradiance, 3m floats. */
+int rn_photon_map_gather(const rn_photon_map *map, int m, const float *points, const float *normals, float radius,
+                         float *irradiance);
+
+/* Pass 2 (src/Renderer.cpp:1328-1345, photonShade :1233-1294) with rn_render's
+ * camera (src/capi.cpp:49-50): direct light plus the photon irradiance at the
+ * first diffuse hit through mirrors, metal and glass (depth 4), gathered
+ * within the map's cell side; gather_rays > 0 (at most 4096) gathers one
+ * bounce away instead (final gather). RGBA8 as rn_render writes it. */
+int rn_render_photon(const rn_scene *scene, const rn_photon_map *map, int width, int height, float cam_z,
+                     int gather_rays, uint32_t key0, uint32_t key1, unsigned char *rgba);
+
+/* Bidirectional path tracing, renderBidirectional (src/BDPT.cpp:66-144), with
+ * rn_render's camera: per sample a camera and a light subpath of up to 4
+ * diffuse bounces each, the fixed ceiling light of src/BDPT.cpp:77-83, every
+ * vertex pair connected with a shadow ray and weighted 1 / (i + j + 2), then
+ * exposure and Reinhard. The reference's per-row std::mt19937 (seed 9871 + y)
+ * becomes the Philox stream (pixel, sample, 'bdpt'). samples in [1, 2^20]. */
+int rn_render_bidirectional(const rn_scene *scene, int width, int height, float cam_z, int samples, float exposure,
+                            uint32_t key0, uint32_t key1, unsigned char *rgba);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif

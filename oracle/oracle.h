@@ -44,6 +44,8 @@ int or_compat_intersect_tri(const float o[3], const float d[3], const float v0[3
  * at the origin, f = 300 px.
  * [pinned: tests/tests.cpp:113-148 projection/orthonormality; goldens.] */
 void or_compat_primary_ray(int x, int y, int W, int H, float cam_z, float jx, float jy, float o[3], float d[3]);
+/* The same camera for a screen offset (sx, sy) from the image centre, pixels. */
+void or_compat_camera_ray(float sx, float sy, float cam_z, float o[3], float d[3]);
 
 /* Brute-force closest hit (src/Geometry.cpp:45-63): first minimum in index
  * order, strict <. verts = 9n floats. Returns prim or -1. */
@@ -89,6 +91,28 @@ typedef struct {
 
 void or_compat_render(const or_compat_scene *scene, int W, int H, float cam_z, int spp, uint32_t key0,
                       uint32_t key1, int y0, int y1, unsigned char *rgba);
+
+/* Photon mapping and bidirectional path tracing (photon.c). `rng` selects
+ * the reference's std::mt19937 streams (OR_RNG_MT; the images then equal the
+ * compiled reference's) or the GPU's Philox layout (OR_RNG_PHILOX).
+ * [pinned: OR_RNG_MT images against renderPhotonMapped / renderBidirectional
+ *  of the compiled reference, pixel for pixel; or_photon_gather against
+ *  PhotonMap::gather.] */
+enum { OR_RNG_MT = 0, OR_RNG_PHILOX = 1 };
+/* Pass 1 of renderPhotonMapped: returns the photon count (arrays 3 floats
+ * per photon, at most `cap` written). */
+int or_photon_trace(const or_compat_scene *scene, int num_photons, int rng, uint32_t key0, uint32_t key1,
+                    float *pos, float *power, float *dir, int cap);
+/* PhotonMap::build(cell) + gather(point, normal, radius) at m points. */
+void or_photon_gather(int n, const float *pos, const float *power, const float *dir, float cell, int m,
+                      const float *points, const float *normals, float radius, float *out);
+/* Pass 2 of renderPhotonMapped with rn_render's camera over rows [y0, y1). */
+void or_photon_render(const or_compat_scene *scene, int n, const float *pos, const float *power, const float *dir,
+                      int W, int H, float cam_z, int gather_rays, int rng, uint32_t key0, uint32_t key1, int y0,
+                      int y1, unsigned char *rgba);
+/* renderBidirectional with rn_render's camera over rows [y0, y1). */
+void or_bdpt_render(const or_compat_scene *scene, int W, int H, float cam_z, int samples, float exposure, int rng,
+                    uint32_t key0, uint32_t key1, int y0, int y1, unsigned char *rgba);
 
 /* ------------------------------------------------------------------ LBVH -- */
 

@@ -82,6 +82,15 @@ def lib():
         L.or_fractal_noise.restype = C.c_float
         L.or_compat_render.argtypes = [C.POINTER(CompatScene), C.c_int, C.c_int, C.c_float, C.c_int, C.c_uint32,
                                        C.c_uint32, C.c_int, C.c_int, u8p]
+        L.or_compat_camera_ray.argtypes = [C.c_float, C.c_float, C.c_float, f32p, f32p]
+        L.or_photon_trace.argtypes = [C.POINTER(CompatScene), C.c_int, C.c_int, C.c_uint32, C.c_uint32, f32p, f32p,
+                                      f32p, C.c_int]
+        L.or_photon_trace.restype = C.c_int
+        L.or_photon_gather.argtypes = [C.c_int, f32p, f32p, f32p, C.c_float, C.c_int, f32p, f32p, C.c_float, f32p]
+        L.or_photon_render.argtypes = [C.POINTER(CompatScene), C.c_int, f32p, f32p, f32p, C.c_int, C.c_int,
+                                       C.c_float, C.c_int, C.c_int, C.c_uint32, C.c_uint32, C.c_int, C.c_int, u8p]
+        L.or_bdpt_render.argtypes = [C.POINTER(CompatScene), C.c_int, C.c_int, C.c_float, C.c_int, C.c_float,
+                                     C.c_int, C.c_uint32, C.c_uint32, C.c_int, C.c_int, u8p]
         L.or_morton3.argtypes = [C.c_uint32] * 3
         L.or_morton3.restype = C.c_uint32
         L.or_morton_codes.argtypes = [f32p, C.c_int, u32p]
@@ -122,11 +131,91 @@ def ref():
         L.rn_scene_free.argtypes = [C.c_void_p]
         L.rn_render.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int, u8p]
         L.rn_render.restype = C.c_int
+        L.ref_render_photon.argtypes = [C.c_char_p, C.c_float, C.c_int, C.c_int, C.c_float, C.c_int, C.c_int, u8p]
+        L.ref_render_photon.restype = C.c_int
+        L.ref_render_bidirectional.argtypes = [C.c_char_p, C.c_float, C.c_int, C.c_int, C.c_float, C.c_int,
+                                               C.c_float, u8p]
+        L.ref_render_bidirectional.restype = C.c_int
+        L.ref_photon_gather.argtypes = [C.c_int, f32p, f32p, f32p, C.c_float, C.c_int, f32p, f32p, C.c_float, f32p]
         _ref = L
     return _ref
 
 
 # ----------------------------------------------------------------- helpers --
+
+RNG_MT, RNG_PHILOX = 0, 1
+
+
+class CompatArrays:
+    """An or_compat_scene over numpy arrays (kept alive by this object):
+    verts (n,9), normals (n,3), colour (n,3) ints, material (n,), texture_id
+    (n,), texcoords (n,6) and a list of ARGB texture images."""
+
+    def __init__(self, verts, normals, colour, material, texture_id, texcoords, textures=()):
+        self._keep = []
+        cs = CompatScene()
+
+        def arr(a, dt):
+            a = np.ascontiguousarray(a, dtype=dt).reshape(-1)
+            self._keep.append(a)
+            return a.ctypes.data
+
+        cs.n = int(np.asarray(material).size)
+        cs.verts = arr(verts, np.float32)
+        cs.normals = arr(normals, np.float32)
+        cs.colour = arr(colour, np.int32)
+        cs.material = arr(material, np.int32)
+        cs.texture_id = arr(texture_id, np.int32)
+        cs.texcoords = arr(texcoords, np.float32)
+        cs.n_textures = len(textures)
+        if textures:
+            cs.tex_w = arr([t.shape[1] for t in textures], np.int32)
+            cs.tex_h = arr([t.shape[0] for t in textures], np.int32)
+            ptrs = (C.c_void_p * len(textures))(*[arr(t, np.uint32) for t in textures])
+            self._keep.append(ptrs)
+            cs.tex = C.cast(ptrs, C.c_void_p)
+        self.s = cs
+
+    @classmethod
+    def from_scene(cls, sc):
+        """From a rednoise_amd.Scene (its host-side arrays)."""
+        ca = sc.compat_attributes()
+        v, _, _ = sc.triangles()
+        ntex = int(ca["texture_id"].max()) + 1 if ca["texture_id"].size else 0
+        return cls(v, ca["normals"], ca["colour"], ca["material"], ca["texture_id"], ca["texcoords"],
+                   [sc.texture(i) for i in range(ntex)])
+
+
+def photon_trace(cs: CompatArrays, num_photons: int, rng: int, key=(0, 0)):
+    """Pass 1 of the photon mapper: (pos, power, dir), each (count, 3)."""
+    L = lib()
+    cap = 6 * num_photons
+    pos, pw, dr = (np.empty(3 * cap, np.float32) for _ in range(3))
+    n = L.or_photon_trace(C.byref(cs.s), num_photons, rng, key[0], key[1], pos, pw, dr, cap)
+    return pos[:3 * n].reshape(-1, 3), pw[:3 * n].reshape(-1, 3), dr[:3 * n].reshape(-1, 3)
+
+
+def photon_render(cs: CompatArrays, photons, W, H, cam_z, gather_rays=0, rng=RNG_MT, key=(0, 0)):
+    pos, pw, dr = (np.ascontiguousarray(a, np.float32).reshape(-1) for a in photons)
+    out = np.zeros(W * H * 4, np.uint8)
+    lib().or_photon_render(C.byref(cs.s), pos.size // 3, pos, pw, dr, W, H, cam_z, gather_rays, rng, key[0], key[1],
+                           0, H, out)
+    return out.reshape(H, W, 4)
+
+
+def bdpt_render(cs: CompatArrays, W, H, cam_z, samples=64, exposure=40.0, rng=RNG_MT, key=(0, 0)):
+    out = np.zeros(W * H * 4, np.uint8)
+    lib().or_bdpt_render(C.byref(cs.s), W, H, cam_z, samples, exposure, rng, key[0], key[1], 0, H, out)
+    return out.reshape(H, W, 4)
+
+
+def photon_gather(photons, cell, points, normals, radius):
+    pos, pw, dr = (np.ascontiguousarray(a, np.float32).reshape(-1) for a in photons)
+    pts = np.ascontiguousarray(points, np.float32).reshape(-1)
+    nms = np.ascontiguousarray(normals, np.float32).reshape(-1)
+    out = np.empty(pts.size, np.float32)
+    lib().or_photon_gather(pos.size // 3, pos, pw, dr, cell, pts.size // 3, pts, nms, radius, out)
+    return out.reshape(-1, 3)
 
 def philox(ctr: np.ndarray, key: np.ndarray) -> np.ndarray:
     ctr = np.ascontiguousarray(ctr, dtype=np.uint32).reshape(-1, 4)

@@ -3,11 +3,15 @@
 // Makefile). Used only by tests/golden/make_golden.py to write golden vectors
 // and by the test-suite when the library is present. The reference code is
 // called, never modified or copied.
+#include "BDPT.h"
 #include "BVH.h"
 #include "Camera.h"
 #include "Geometry.h"
 #include "Noise.h"
 #include "ObjLoader.h"
+#include "Photon.h"
+#include "Renderer.h"
+#include <Canvas.h>
 #include <ModelTriangle.h>
 #include <cstdint>
 #include <cstring>
@@ -154,6 +158,67 @@ void ref_primary_rays(int w, int h, float cam_z, const float *jitter, float *o, 
 void ref_fractal_noise(int np, const float *p, int octaves, float *out) {
 	for (int i = 0; i < np; i++)
 		out[i] = fractalNoise(glm::vec3(p[3 * i], p[3 * i + 1], p[3 * i + 2]), octaves);
+}
+
+// Canvas 0xAARRGGBB -> RGBA8 as rn_render unpacks it (src/capi.cpp:68-76).
+static void unpack(const Canvas &canvas, int count, unsigned char *rgba) {
+	for (int i = 0; i < count; i++) {
+		uint32_t p = canvas.pixels[static_cast<size_t>(i)];
+		rgba[i * 4 + 0] = static_cast<unsigned char>((p >> 16) & 0xFF);
+		rgba[i * 4 + 1] = static_cast<unsigned char>((p >> 8) & 0xFF);
+		rgba[i * 4 + 2] = static_cast<unsigned char>(p & 0xFF);
+		rgba[i * 4 + 3] = static_cast<unsigned char>((p >> 24) & 0xFF);
+	}
+}
+
+// renderPhotonMapped (src/Renderer.cpp:1297-1346) of an OBJ with rn_render's
+// camera (src/capi.cpp:49-50) and the default light.
+int ref_render_photon(const char *path, float scale, int w, int h, float cam_z, int num_photons, int gather_rays,
+                      unsigned char *rgba) {
+	std::vector<ModelTriangle> model = loadOBJ(path, scale);
+	if (model.empty())
+		return 0;
+	Camera camera(w, h, 2.0f, glm::vec3(0.0f, 0.0f, cam_z));
+	camera.lookAt(glm::vec3(0.0f));
+	Canvas canvas(w, h);
+	renderPhotonMapped(model, camera, canvas, num_photons, {}, {}, gather_rays);
+	unpack(canvas, w * h, rgba);
+	return 1;
+}
+
+// renderBidirectional (src/BDPT.cpp:66-144) of an OBJ with rn_render's camera.
+int ref_render_bidirectional(const char *path, float scale, int w, int h, float cam_z, int samples, float exposure,
+                             unsigned char *rgba) {
+	std::vector<ModelTriangle> model = loadOBJ(path, scale);
+	if (model.empty())
+		return 0;
+	Camera camera(w, h, 2.0f, glm::vec3(0.0f, 0.0f, cam_z));
+	camera.lookAt(glm::vec3(0.0f));
+	Canvas canvas(w, h);
+	renderBidirectional(model, camera, canvas, samples, exposure);
+	unpack(canvas, w * h, rgba);
+	return 1;
+}
+
+// PhotonMap::build(cell) then gather(point, normal, radius) at m points
+// (src/Photon.cpp:10-46).
+void ref_photon_gather(int n, const float *pos, const float *power, const float *dir, float cell, int m,
+                       const float *points, const float *normals, float radius, float *out) {
+	PhotonMap map;
+	map.photons.resize(static_cast<size_t>(n));
+	for (int i = 0; i < n; i++)
+		for (int a = 0; a < 3; a++) {
+			map.photons[static_cast<size_t>(i)].position[a] = pos[3 * i + a];
+			map.photons[static_cast<size_t>(i)].power[a] = power[3 * i + a];
+			map.photons[static_cast<size_t>(i)].direction[a] = dir[3 * i + a];
+		}
+	map.build(cell);
+	for (int i = 0; i < m; i++) {
+		glm::vec3 g = map.gather(glm::vec3(points[3 * i], points[3 * i + 1], points[3 * i + 2]),
+		                         glm::vec3(normals[3 * i], normals[3 * i + 1], normals[3 * i + 2]), radius);
+		for (int a = 0; a < 3; a++)
+			out[3 * i + a] = g[a];
+	}
 }
 
 } // extern "C"

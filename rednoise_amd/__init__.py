@@ -193,6 +193,12 @@ def _configure(lib: C.CDLL) -> None:
         "rn_scene_set_builder": ([vp, i], i),
         "rn_lbvh_stages": ([vp, i, vp, vp, vp, vp], i), "rn_radix_sort_pairs": ([vp, vp, i, i], i),
         "rn_philox4x32_10": ([i, vp, vp, vp], i),
+        "rn_photon_map_trace": ([vp, i, C.c_uint32, C.c_uint32], vp),
+        "rn_photon_map_create": ([i, vp, vp, vp, f], vp), "rn_photon_map_free": ([vp], None),
+        "rn_photon_map_count": ([vp], i), "rn_photon_map_get": ([vp, vp, vp, vp], i),
+        "rn_photon_map_gather": ([vp, i, vp, vp, f, vp], i),
+        "rn_render_photon": ([vp, vp, i, i, f, i, C.c_uint32, C.c_uint32, vp], i),
+        "rn_render_bidirectional": ([vp, i, i, f, i, f, C.c_uint32, C.c_uint32, vp], i),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -259,6 +265,70 @@ class Accumulator:
     def close(self) -> None:
         if getattr(self, "_h", None):
             self._lib.rn_accum_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class PhotonMap:
+    """A photon map on the GPU (``rn_photon_map_*``), the PhotonMap of the
+    reference's photon mapper: ``PhotonMap.trace(scene, n)`` runs pass 1 of
+    renderPhotonMapped, ``PhotonMap.from_photons`` wraps given photons;
+    ``photons()`` returns them in emission order and ``gather`` evaluates
+    PhotonMap::gather."""
+
+    def __init__(self, handle):
+        self._lib = _load_library()
+        self._h = handle
+
+    @classmethod
+    def trace(cls, scene: "Scene", num_photons: int, key0: int = 0, key1: int = 0) -> "PhotonMap":
+        h = _load_library().rn_photon_map_trace(scene.handle, int(num_photons), int(key0) & 0xFFFFFFFF,
+                                                 int(key1) & 0xFFFFFFFF)
+        if not h:
+            raise _err("rn_photon_map_trace")
+        return cls(h)
+
+    @classmethod
+    def from_photons(cls, position, power, direction, cell: float = 0.25) -> "PhotonMap":
+        pos = _f32(position)
+        n = pos.size // 3
+        pw, dr = _f32(power, 3 * n), _f32(direction, 3 * n)
+        h = _load_library().rn_photon_map_create(n, _ptr(pos), _ptr(pw), _ptr(dr), C.c_float(cell))
+        if not h:
+            raise _err("rn_photon_map_create")
+        return cls(h)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def __len__(self) -> int:
+        return int(self._lib.rn_photon_map_count(self._h))
+
+    def photons(self) -> Tuple[np.ndarray, np.ndarray, np.ndarray]:
+        n = len(self)
+        pos, pw, dr = (np.empty(3 * n, np.float32) for _ in range(3))
+        if self._lib.rn_photon_map_get(self._h, _ptr(pos), _ptr(pw), _ptr(dr)) != 1:
+            raise _err("rn_photon_map_get")
+        return pos.reshape(-1, 3), pw.reshape(-1, 3), dr.reshape(-1, 3)
+
+    def gather(self, points, normals, radius: float = 0.25) -> np.ndarray:
+        p = _f32(points)
+        m = p.size // 3
+        q = _f32(normals, 3 * m)
+        out = np.empty(3 * m, np.float32)
+        if self._lib.rn_photon_map_gather(self._h, m, _ptr(p), _ptr(q), C.c_float(radius), _ptr(out)) != 1:
+            raise _err("rn_photon_map_gather")
+        return out.reshape(-1, 3)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.rn_photon_map_free(self._h)
             self._h = None
 
     def __del__(self):
@@ -424,6 +494,28 @@ class Scene:
         if _load_library().rn_render_seeded(self._handle, width, height, C.c_float(cam_z), int(samples),
                                             int(key0) & 0xFFFFFFFF, int(key1) & 0xFFFFFFFF, _ptr(buf)) != 1:
             raise _err("rn_render_seeded")
+        return Image(buf.tobytes(), width, height)
+
+    def render_photon(self, photon_map: "PhotonMap", width: int, height: int, cam_z: float, gather_rays: int = 0,
+                      key0: int = 0, key1: int = 0) -> Image:
+        """``rn_render_photon``: pass 2 of the reference's photon mapper."""
+        self._check_open()
+        buf = np.empty(width * height * 4, np.uint8)
+        if _load_library().rn_render_photon(self._handle, photon_map.handle, width, height, C.c_float(cam_z),
+                                            int(gather_rays), int(key0) & 0xFFFFFFFF, int(key1) & 0xFFFFFFFF,
+                                            _ptr(buf)) != 1:
+            raise _err("rn_render_photon")
+        return Image(buf.tobytes(), width, height)
+
+    def render_bidirectional(self, width: int, height: int, cam_z: float, samples: int = 64, exposure: float = 40.0,
+                             key0: int = 0, key1: int = 0) -> Image:
+        """``rn_render_bidirectional``: the reference's bidirectional path tracer."""
+        self._check_open()
+        buf = np.empty(width * height * 4, np.uint8)
+        if _load_library().rn_render_bidirectional(self._handle, width, height, C.c_float(cam_z), int(samples),
+                                                   C.c_float(exposure), int(key0) & 0xFFFFFFFF,
+                                                   int(key1) & 0xFFFFFFFF, _ptr(buf)) != 1:
+            raise _err("rn_render_bidirectional")
         return Image(buf.tobytes(), width, height)
 
     def render_hdr(self, camera: Camera, params: RenderParams) -> np.ndarray:

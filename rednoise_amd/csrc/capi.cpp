@@ -8,6 +8,7 @@
 #include "rn_internal.h"
 #include "rn_kernels.h"
 #include "scene_io.h"
+#include "photon.h"
 #include "wavefront.h"
 
 #include <algorithm>
@@ -325,6 +326,10 @@ DevParams make_params(const rn_render_params &p, const rn_scene *sc, uint32_t sc
 }
 
 } // namespace rn
+
+struct rn_photon_map {
+	PhotonMap m;
+};
 
 #define RN_TRY try {
 #define RN_CATCH(ret)                                                                                                  \
@@ -1167,6 +1172,212 @@ int rn_philox4x32_10(int n, const uint32_t *ctr, const uint32_t *key, uint32_t *
 	launch_philox(n, dc, dk, dout, c.stream);
 	download<uint32_t>(out, dout, 4 * N, c.stream);
 	RN_HIP(hipStreamSynchronize(c.stream));
+	return 1;
+	RN_CATCH(0)
+}
+
+// ---------------------------------------------------- photon mapping --
+
+rn_photon_map *rn_photon_map_trace(const rn_scene *scene, int num_photons, uint32_t key0, uint32_t key1) {
+	RN_TRY
+	t_error.clear();
+	if (!scene)
+		throw Error("rn_photon_map_trace: scene is NULL");
+	if (num_photons < 0 || num_photons > (1 << 26))
+		throw Error("rn_photon_map_trace: photon count must be in [0, 2^26]");
+	auto t0 = std::chrono::steady_clock::now();
+	Context &c = ctx();
+	std::lock_guard<std::mutex> lk(c.mu);
+	rn_scene *sc = const_cast<rn_scene *>(scene);
+	std::lock_guard<std::mutex> sl(sc->mu);
+	sc->ensure_bvh(8, c.stream);
+	std::unique_ptr<rn_photon_map> map(new rn_photon_map());
+	RN_HIP(hipEventRecord(c.ev0, c.stream));
+	photon_trace(map->m, sc->dev(), sc->compat(), num_photons, key0, key1, c.stream);
+	RN_HIP(hipEventRecord(c.ev1, c.stream));
+	RN_HIP(hipStreamSynchronize(c.stream));
+	float ms = 0;
+	RN_HIP(hipEventElapsedTime(&ms, c.ev0, c.ev1));
+	std::memset(&t_stats, 0, sizeof t_stats);
+	t_stats.render_ms = ms;
+	t_stats.total_ms = ms_since(t0);
+	return map.release();
+	RN_CATCH(nullptr)
+}
+
+rn_photon_map *rn_photon_map_create(int n, const float *position, const float *power, const float *direction,
+                                    float cell) {
+	RN_TRY
+	t_error.clear();
+	if (n < 0 || (n > 0 && (!position || !power || !direction)))
+		throw Error("rn_photon_map_create: bad photon arrays");
+	if (!(cell > 0.0f) || !std::isfinite(cell))
+		throw Error("rn_photon_map_create: cell side must be positive and finite");
+	Context &c = ctx();
+	std::lock_guard<std::mutex> lk(c.mu);
+	std::unique_ptr<rn_photon_map> map(new rn_photon_map());
+	PhotonMap &m = map->m;
+	m.n = n;
+	size_t N = (size_t)std::max(n, 1);
+	std::vector<float4> a(N), b(N);
+	std::vector<float> cc(N);
+	for (size_t i = 0; i < (size_t)n; i++) {
+		a[i] = make_float4(position[3 * i], position[3 * i + 1], position[3 * i + 2], power[3 * i]);
+		b[i] = make_float4(direction[3 * i], direction[3 * i + 1], direction[3 * i + 2], power[3 * i + 1]);
+		cc[i] = power[3 * i + 2];
+	}
+	upload(m.a, a.data(), (size_t)n, c.stream);
+	upload(m.b, b.data(), (size_t)n, c.stream);
+	upload(m.c, cc.data(), (size_t)n, c.stream);
+	if (n == 0) {
+		m.a.reserve(16);
+		m.b.reserve(16);
+		m.c.reserve(4);
+	}
+	photon_map_build(m, cell, c.stream);
+	RN_HIP(hipStreamSynchronize(c.stream));
+	return map.release();
+	RN_CATCH(nullptr)
+}
+
+void rn_photon_map_free(rn_photon_map *map) {
+	if (!map)
+		return;
+	try {
+		Context &c = ctx();
+		std::lock_guard<std::mutex> lk(c.mu);
+		RN_HIP(hipStreamSynchronize(c.stream));
+		delete map;
+	} catch (...) {
+	}
+}
+
+int rn_photon_map_count(const rn_photon_map *map) { return map ? map->m.n : 0; }
+
+int rn_photon_map_get(const rn_photon_map *map, float *position, float *power, float *direction) {
+	RN_TRY
+	t_error.clear();
+	if (!map)
+		throw Error("rn_photon_map_get: map is NULL");
+	Context &c = ctx();
+	std::lock_guard<std::mutex> lk(c.mu);
+	const PhotonMap &m = map->m;
+	size_t n = (size_t)m.n;
+	std::vector<float4> a(n), b(n);
+	std::vector<float> cc(n);
+	download<float4>(a.data(), m.a.ptr, n, c.stream);
+	download<float4>(b.data(), m.b.ptr, n, c.stream);
+	download<float>(cc.data(), m.c.ptr, n, c.stream);
+	RN_HIP(hipStreamSynchronize(c.stream));
+	for (size_t i = 0; i < n; i++) {
+		if (position) {
+			position[3 * i] = a[i].x;
+			position[3 * i + 1] = a[i].y;
+			position[3 * i + 2] = a[i].z;
+		}
+		if (power) {
+			power[3 * i] = a[i].w;
+			power[3 * i + 1] = b[i].w;
+			power[3 * i + 2] = cc[i];
+		}
+		if (direction) {
+			direction[3 * i] = b[i].x;
+			direction[3 * i + 1] = b[i].y;
+			direction[3 * i + 2] = b[i].z;
+		}
+	}
+	return 1;
+	RN_CATCH(0)
+}
+
+int rn_photon_map_gather(const rn_photon_map *map, int m, const float *points, const float *normals, float radius,
+                         float *irradiance) {
+	RN_TRY
+	t_error.clear();
+	if (!map || m < 0 || (m > 0 && (!points || !normals || !irradiance)))
+		throw Error("rn_photon_map_gather: bad arguments");
+	if (!(radius >= 0.0f) || !std::isfinite(radius))
+		throw Error("rn_photon_map_gather: radius must be finite and non-negative");
+	if (m == 0)
+		return 1;
+	Context &c = ctx();
+	std::lock_guard<std::mutex> lk(c.mu);
+	const float *dp = upload(c.in0, points, 3 * (size_t)m, c.stream);
+	const float *dn = upload(c.in1, normals, 3 * (size_t)m, c.stream);
+	float *dout = static_cast<float *>(c.o0.reserve(3 * (size_t)m * 4));
+	launch_photon_gather(map->m, m, dp, dn, radius, dout, c.stream);
+	download<float>(irradiance, dout, 3 * (size_t)m, c.stream);
+	RN_HIP(hipStreamSynchronize(c.stream));
+	return 1;
+	RN_CATCH(0)
+}
+
+static void check_image(const rn_scene *scene, int width, int height, const unsigned char *rgba, const char *who) {
+	if (!scene || !rgba || width <= 0 || height <= 0)
+		throw Error(std::string(who) + ": NULL scene/buffer or non-positive size");
+	if ((long long)width * height > (1ll << 29))
+		throw Error(std::string(who) + ": image too large");
+}
+
+int rn_render_photon(const rn_scene *scene, const rn_photon_map *map, int width, int height, float cam_z,
+                     int gather_rays, uint32_t key0, uint32_t key1, unsigned char *rgba) {
+	RN_TRY
+	t_error.clear();
+	check_image(scene, width, height, rgba, "rn_render_photon");
+	if (!map)
+		throw Error("rn_render_photon: map is NULL");
+	if (gather_rays < 0 || gather_rays > 4096)
+		throw Error("rn_render_photon: gather_rays must be in [0, 4096]");
+	auto t0 = std::chrono::steady_clock::now();
+	Context &c = ctx();
+	std::lock_guard<std::mutex> lk(c.mu);
+	rn_scene *sc = const_cast<rn_scene *>(scene);
+	std::lock_guard<std::mutex> sl(sc->mu);
+	sc->ensure_bvh(8, c.stream);
+	unsigned char *d_rgba = static_cast<unsigned char *>(c.out.reserve((size_t)width * height * 4));
+	RN_HIP(hipEventRecord(c.ev0, c.stream));
+	launch_photon_render(sc->dev(), sc->compat(), map->m, width, height, cam_z, gather_rays, key0, key1, d_rgba,
+	                     c.stream);
+	RN_HIP(hipEventRecord(c.ev1, c.stream));
+	download<unsigned char>(rgba, d_rgba, (size_t)width * height * 4, c.stream);
+	RN_HIP(hipStreamSynchronize(c.stream));
+	float ms = 0;
+	RN_HIP(hipEventElapsedTime(&ms, c.ev0, c.ev1));
+	std::memset(&t_stats, 0, sizeof t_stats);
+	t_stats.render_ms = ms;
+	t_stats.total_ms = ms_since(t0);
+	t_stats.trace_kernel_ms = ms;
+	t_stats.trace_kernel_launches = 1;
+	return 1;
+	RN_CATCH(0)
+}
+
+int rn_render_bidirectional(const rn_scene *scene, int width, int height, float cam_z, int samples, float exposure,
+                            uint32_t key0, uint32_t key1, unsigned char *rgba) {
+	RN_TRY
+	t_error.clear();
+	check_image(scene, width, height, rgba, "rn_render_bidirectional");
+	if (samples <= 0 || samples > (1 << 20))
+		throw Error("rn_render_bidirectional: samples must be in [1, 2^20]");
+	auto t0 = std::chrono::steady_clock::now();
+	Context &c = ctx();
+	std::lock_guard<std::mutex> lk(c.mu);
+	rn_scene *sc = const_cast<rn_scene *>(scene);
+	std::lock_guard<std::mutex> sl(sc->mu);
+	sc->ensure_bvh(8, c.stream);
+	unsigned char *d_rgba = static_cast<unsigned char *>(c.out.reserve((size_t)width * height * 4));
+	RN_HIP(hipEventRecord(c.ev0, c.stream));
+	launch_bdpt_render(sc->dev(), sc->compat(), width, height, cam_z, samples, exposure, key0, key1, d_rgba, c.stream);
+	RN_HIP(hipEventRecord(c.ev1, c.stream));
+	download<unsigned char>(rgba, d_rgba, (size_t)width * height * 4, c.stream);
+	RN_HIP(hipStreamSynchronize(c.stream));
+	float ms = 0;
+	RN_HIP(hipEventElapsedTime(&ms, c.ev0, c.ev1));
+	std::memset(&t_stats, 0, sizeof t_stats);
+	t_stats.render_ms = ms;
+	t_stats.total_ms = ms_since(t0);
+	t_stats.trace_kernel_ms = ms;
+	t_stats.trace_kernel_launches = 1;
 	return 1;
 	RN_CATCH(0)
 }

@@ -799,6 +799,12 @@ void lbvh_front(Bvh &b, const float *d_verts, int n, hipStream_t s, bool karras 
 
 } // namespace
 
+void device_exclusive_scan(const uint32_t *in, uint32_t *out, size_t n, uint32_t *tmp, hipStream_t s) {
+	exclusive_scan(in, out, n, tmp, s);
+}
+
+size_t device_scan_tmp_words(size_t n) { return scan_tmp_words(n); }
+
 void radix_sort_pairs(uint32_t *d_keys, uint32_t *d_vals, uint32_t *d_keys_tmp, uint32_t *d_vals_tmp, int n,
                       int bits, DevBuf &hist, DevBuf &scan_tmp, hipStream_t s) {
 	if (n <= 1)

@@ -44,23 +44,52 @@ RN_DEV float3 gcross(float3 x, float3 y) {
 #pragma clang fp contract(off)
 	return f3(x.y * y.z - y.y * x.z, x.z * y.x - y.z * x.x, x.x * y.y - y.x * x.y);
 }
+// Component-wise vector arithmetic that is never fused into multiply-adds
+// (the float3 operators may be, once inlined into a caller).
+RN_DEV float3 gadd(float3 a, float3 b) {
+#pragma clang fp contract(off)
+	return f3(a.x + b.x, a.y + b.y, a.z + b.z);
+}
+RN_DEV float3 gsub(float3 a, float3 b) {
+#pragma clang fp contract(off)
+	return f3(a.x - b.x, a.y - b.y, a.z - b.z);
+}
+RN_DEV float3 gmul(float3 a, float3 b) {
+#pragma clang fp contract(off)
+	return f3(a.x * b.x, a.y * b.y, a.z * b.z);
+}
+RN_DEV float3 gscale(float3 a, float s) {
+#pragma clang fp contract(off)
+	return f3(a.x * s, a.y * s, a.z * s);
+}
+RN_DEV float3 gdiv(float3 a, float s) {
+#pragma clang fp contract(off)
+	return f3(a.x / s, a.y / s, a.z / s);
+}
 
 // rn_render's camera: Camera(w,h,2,(0,0,cam_z)).lookAt(0) then primaryRay
-// (src/capi.cpp:49-50, src/Camera.cpp:27-33,69-79, src/Renderer.cpp:1148).
-RN_DEV void compat_primary_ray(int x, int y, int W, int H, float cam_z, float jx, float jy, float3 &o, float3 &d) {
+// (src/capi.cpp:49-50, src/Camera.cpp:27-33,69-79) through the screen offset
+// (sx, sy) pixels from the image centre.
+RN_DEV void compat_camera_ray(float sx, float sy, float cam_z, float3 &o, float3 &d) {
 #pragma clang fp contract(off)
 	float3 pos = f3(0.0f, 0.0f, cam_z);
 	float3 fwd = gnormalize(f3(pos.x - 0.0f, pos.y - 0.0f, pos.z - 0.0f));
 	float3 right = gnormalize(gcross(f3(0.0f, 1.0f, 0.0f), fwd));
 	float3 up = gcross(fwd, right);
 	float f = 2.0f * 150.0f;
-	float sx = (float)x + 0.5f + jx - (float)W / 2.0f;
-	float sy = -((float)y + 0.5f + jy - (float)H / 2.0f);
 	float a = sx / f, b = sy / f, c = -1.0f;
 	float3 dir = f3(right.x * a + up.x * b + fwd.x * c, right.y * a + up.y * b + fwd.y * c,
 	                right.z * a + up.z * b + fwd.z * c);
 	o = pos;
 	d = gnormalize(dir);
+}
+
+// The pixel (x, y) with jitter (jx, jy) (src/Renderer.cpp:1148).
+RN_DEV void compat_primary_ray(int x, int y, int W, int H, float cam_z, float jx, float jy, float3 &o, float3 &d) {
+#pragma clang fp contract(off)
+	float sx = (float)x + 0.5f + jx - (float)W / 2.0f;
+	float sy = -((float)y + 0.5f + jy - (float)H / 2.0f);
+	compat_camera_ray(sx, sy, cam_z, o, d);
 }
 
 // Barycentric texture coordinate (src/Renderer.cpp:642-649).

@@ -61,5 +61,8 @@ void lbvh_stages(const float *d_verts, int n, uint32_t *h_codes, int *h_order, i
                  hipStream_t s);
 void radix_sort_pairs(uint32_t *d_keys, uint32_t *d_vals, uint32_t *d_keys_tmp, uint32_t *d_vals_tmp, int n,
                       int bits, DevBuf &hist, DevBuf &scan_tmp, hipStream_t s);
+// Exclusive prefix sum of n words; tmp holds device_scan_tmp_words(n) words.
+void device_exclusive_scan(const uint32_t *in, uint32_t *out, size_t n, uint32_t *tmp, hipStream_t s);
+size_t device_scan_tmp_words(size_t n);
 
 } // namespace rn

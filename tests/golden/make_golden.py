@@ -15,6 +15,11 @@ Outputs (small .npz fixtures next to this script):
 With --whitted (only these files):
   whitted_<asset>.npz  rn_render(RN_RAYTRACED) RGBA8 images (160x120, 96x64,
                        cam_z 4) and loadOBJ's smoothed vertex normals
+With --photon (only these files):
+  photon_<asset>.npz   renderPhotonMapped (20000 photons; no final gather and
+                       3 gather rays) and renderBidirectional (16 samples,
+                       exposure 40) RGBA8 images at 64x48 with rn_render's
+                       camera (cam_z 4)
 """
 from __future__ import annotations
 
@@ -96,6 +101,28 @@ def whitted(ref) -> None:
         print("whitted", key, {k: v.shape for k, v in out.items()})
 
 
+PHOTON = {"cornell": ("cornell-box.obj", 0.35), "quad": ("textured-quad.obj", 0.35), "sphere": ("sphere.obj", 1.0)}
+
+
+def photon(ref) -> None:
+    """renderPhotonMapped / renderBidirectional images of the compiled
+    reference per asset: photon_<asset>.npz."""
+    W, H, cz, nph, spp, expo = 64, 48, 4.0, 20000, 16, 40.0
+    for key, (name, scale) in PHOTON.items():
+        path = os.path.join(ASSETS, name).encode()
+        out = {"scale": np.float32(scale), "cam_z": np.float32(cz), "num_photons": np.int32(nph),
+               "bdpt_samples": np.int32(spp), "bdpt_exposure": np.float32(expo)}
+        for g in (0, 3):
+            buf = np.empty(W * H * 4, np.uint8)
+            assert ref.ref_render_photon(path, scale, W, H, cz, nph, g, buf) == 1
+            out[f"photon_g{g}"] = buf.reshape(H, W, 4)
+        buf = np.empty(W * H * 4, np.uint8)
+        assert ref.ref_render_bidirectional(path, scale, W, H, cz, spp, expo, buf) == 1
+        out["bdpt"] = buf.reshape(H, W, 4)
+        np.savez_compressed(os.path.join(HERE, f"photon_{key}.npz"), **out)
+        print("photon", key, {k: v.shape for k, v in out.items()})
+
+
 def main() -> None:
     ref = O.ref()
     if ref is None:
@@ -152,10 +179,13 @@ def main() -> None:
 
 
 if __name__ == "__main__":
-    if "--whitted" in sys.argv:
+    if "--whitted" in sys.argv or "--photon" in sys.argv:
         r = O.ref()
         if r is None:
             raise SystemExit("build the reference first: make -C oracle ref")
-        whitted(r)
+        if "--whitted" in sys.argv:
+            whitted(r)
+        if "--photon" in sys.argv:
+            photon(r)
     else:
         main()

@@ -166,3 +166,18 @@ def test_builder_selection_host_side(rn):
     assert lib.rn_scene_set_builder(sc._handle, 7) == 0
     assert b"builder" in lib.rn_last_error()
     assert lib.rn_scene_set_builder(None, 0) == 0
+
+
+def test_photon_and_bidirectional_argument_errors_host_side(rn):
+    import ctypes as C
+    lib = rn._load_library()
+    buf = np.empty(16 * 16 * 4, np.uint8)
+    assert lib.rn_render_photon(None, None, 16, 16, C.c_float(4.0), 0, 0, 0, buf.ctypes.data) == 0
+    assert b"NULL" in lib.rn_last_error()
+    assert lib.rn_render_bidirectional(None, 16, 16, C.c_float(4.0), 4, C.c_float(40.0), 0, 0, buf.ctypes.data) == 0
+    assert lib.rn_photon_map_create(-1, None, None, None, C.c_float(0.25)) is None
+    assert lib.rn_photon_map_create(0, None, None, None, C.c_float(0.0)) is None
+    assert b"cell" in lib.rn_last_error()
+    assert lib.rn_photon_map_count(None) == 0
+    assert lib.rn_photon_map_trace(None, 10, 0, 0) is None
+    lib.rn_photon_map_free(None)
