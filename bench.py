@@ -375,6 +375,59 @@ def reference_modeR(rn, torch):
     return res
 
 
+def estimators_bench(rn):
+    """Photon mapping and bidirectional path tracing on the Cornell asset at
+    the reference CLI's defaults (render -m photon: 640x480, 200000 photons,
+    scale 0.35, cam_z 4), beside the compiled reference (oracle/_ref) when it
+    is present."""
+    from oracle import oracle as O
+    asset = os.path.join(ROOT, "tests", "golden", "assets", "cornell-box.obj")
+    W, H, nph, cz = 640, 480, 200000, 4.0
+    sc = rn.Scene.load_obj(asset, 0.35)
+
+    def best(fn, reps=3):
+        fn()
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t0)
+        return min(ts)
+
+    t_trace = best(lambda: rn.PhotonMap.trace(sc, nph, 0, 0).close())
+    pm = rn.PhotonMap.trace(sc, nph, 0, 0)
+    t_render = best(lambda: sc.render_photon(pm, W, H, cz))
+    t_fg = best(lambda: sc.render_photon(pm, W, H, cz, 4), reps=1)
+    res = {"photon": {"workload": f"cornell-box.obj {W}x{H}, {nph} photons, gather radius 0.25",
+                      "photons_stored": len(pm), "gpu_pass1_s": round(t_trace, 5), "gpu_pass2_s": round(t_render, 5),
+                      "gpu_s": round(t_trace + t_render, 5), "gpu_final_gather4_pass2_s": round(t_fg, 5)}}
+    pm.close()
+    spp = 16
+    t_bd = best(lambda: sc.render_bidirectional(W, H, cz, spp, 40.0))
+    t_bd64 = best(lambda: sc.render_bidirectional(W, H, cz, 64, 40.0), reps=1)
+    res["bidirectional"] = {"workload": f"cornell-box.obj {W}x{H}, {spp} samples, exposure 40",
+                            "gpu_s": round(t_bd, 5), "gpu_64spp_s": round(t_bd64, 5)}
+    ref = O.ref()
+    if ref is not None:
+        buf = np.empty(W * H * 4, np.uint8)
+        t0 = time.perf_counter()
+        ref.ref_render_photon(asset.encode(), 0.35, W, H, cz, nph, 0, buf)
+        t_ref = time.perf_counter() - t0
+        cs = O.CompatArrays.from_scene(sc)
+        mine = sc.render_photon(rn.PhotonMap.from_photons(*O.photon_trace(cs, nph, O.RNG_MT)), W, H, cz).to_numpy()
+        same = float((mine.reshape(-1, 4) == buf.reshape(-1, 4)).all(axis=1).mean())
+        res["photon"].update({"reference_cpu_s": round(t_ref, 3), "reference_threads": os.cpu_count(),
+                              "speedup": round(t_ref / (t_trace + t_render), 1),
+                              "identical_pixels_with_reference_photons": round(same, 5)})
+        buf = np.empty(W * H * 4, np.uint8)
+        t0 = time.perf_counter()
+        ref.ref_render_bidirectional(asset.encode(), 0.35, W, H, cz, spp, 40.0, buf)
+        t_ref = time.perf_counter() - t0
+        res["bidirectional"].update({"reference_cpu_s": round(t_ref, 3), "speedup": round(t_ref / t_bd, 1)})
+    sc.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -519,6 +572,11 @@ def main():
                 result["reference_rn_render"] = r
         except Exception as e:
             result["reference_rn_render"] = {"error": str(e)}
+        try:
+            result["estimators"] = estimators_bench(rn)
+            log(f"estimators: {result['estimators']}")
+        except Exception as e:
+            result["estimators"] = {"error": str(e)}
         result["cpu_baseline"] = cpu_baseline(sd, W, H, spp, depth, rr, eps)
         result["cpu_baseline"]["gpu_over_cpu"] = round(value / result["cpu_baseline"]["value"], 1)
     else:

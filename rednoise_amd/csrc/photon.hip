@@ -26,6 +26,9 @@ namespace {
 constexpr uint32_t kTagPhoton = 0x70686F74u, kTagGather = 0x67617468u, kTagBdpt = 0x62647074u;
 constexpr int kMaxDeposits = 6; // tracePhoton's depth runs 5..0, one deposit at most per level
 constexpr int kPmBlock = 256;
+#ifndef RN_PM_CULL
+#define RN_PM_CULL 1
+#endif
 
 struct Stream {
 	uint32_t a, b, tag, k, k0, k1;
@@ -246,9 +249,23 @@ RN_DEV float3 pm_gather(const PhotonMapDev &m, float3 point, float3 normal, floa
 	const int3 c = cell_of(point, m.cell);
 	float sx = 0.0f, sy = 0.0f, sz = 0.0f;
 	const float r2 = radius * radius;
+	// A neighbour cell whose box (widened by a margin far above the rounding
+	// of a photon's cell index) lies beyond the radius holds no photon that
+	// passes the distance test, so skipping it leaves the sum unchanged.
+	const bool cull = RN_PM_CULL && radius <= 4.0f * m.cell;
+	const float margin = 1e-3f * m.cell + 1e-6f * (fabsf(point.x) + fabsf(point.y) + fabsf(point.z));
+	auto gap = [&](int k, float p) {
+		const float lo = (float)k * m.cell - margin, hi = (float)(k + 1) * m.cell + margin;
+		return p < lo ? lo - p : (p > hi ? p - hi : 0.0f);
+	};
 	for (int dx = -1; dx <= 1; dx++)
 		for (int dy = -1; dy <= 1; dy++)
 			for (int dz = -1; dz <= 1; dz++) {
+				if (cull) {
+					const float gx = gap(c.x + dx, point.x), gy = gap(c.y + dy, point.y), gz = gap(c.z + dz, point.z);
+					if (gx * gx + gy * gy + gz * gz > r2 * 1.0001f)
+						continue;
+				}
 				const unsigned long long key = cell_key(c.x + dx, c.y + dy, c.z + dz);
 				int lo = 0, hi = m.n_cells;
 				while (lo < hi) {
