@@ -27,9 +27,9 @@ extern "C" {
  * (LBVH/BVH8, SoA triangle buffers). Reference: include/rednoise/rednoise.h:21. */
 typedef struct rn_scene rn_scene;
 
-/* Reference: include/rednoise/rednoise.h:24. Only RN_PATHTRACED is implemented
- * by this library; the other modes make rn_render return 0 with an
- * "unsupported mode" error. */
+/* Reference: include/rednoise/rednoise.h:24. RN_PATHTRACED and RN_RAYTRACED
+ * are implemented by this library; the other modes make rn_render return 0
+ * with an "unsupported mode" error. */
 typedef enum { RN_WIREFRAME = 0, RN_RASTERISED = 1, RN_RAYTRACED = 2, RN_PATHTRACED = 3 } rn_render_mode;
 
 /* Load a Wavefront OBJ (+ .mtl, + P6 PPM map_Kd textures), scaling vertices by
@@ -46,9 +46,12 @@ int rn_scene_triangle_count(const rn_scene *scene);
 
 /* Render into the caller's width*height*4 RGBA8 buffer with the fixed camera at
  * (0,0,cam_z) looking at the origin (focal length 300 px). RN_PATHTRACED runs the
- * reference estimator (src/Renderer.cpp:820-907,1119-1170) on the GPU and is
- * synchronous. Returns 1 on success, 0 on a NULL scene/buffer, width/height <= 0,
- * samples <= 0 for RN_PATHTRACED, an unsupported mode or a device error.
+ * reference estimator (src/Renderer.cpp:820-907,1119-1170) on the GPU with the
+ * reference's own random streams (one std::mt19937 per pixel, seeded
+ * (y * W + x) * 9781 + 1), so it computes the reference's image; RN_RAYTRACED
+ * runs the Whitted renderer (src/Renderer.cpp:909-948). Synchronous. Returns 1
+ * on success, 0 on a NULL scene/buffer, width/height <= 0, samples <= 0 for
+ * RN_PATHTRACED, an unsupported mode or a device error.
  * Replaces src/capi.cpp:44-79. */
 int rn_render(const rn_scene *scene, rn_render_mode mode, int width, int height, float cam_z, int samples,
               unsigned char *rgba);

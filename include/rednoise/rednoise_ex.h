@@ -70,9 +70,10 @@ int rn_scene_get_texture(const rn_scene *scene, int id, int *width, int *height,
 
 /* ------------------------------------------------------------ rendering -- */
 
-/* rn_render(scene, RN_PATHTRACED, ...) with an explicit 64-bit Philox key
- * (key0, key1) for the per-pixel sample streams; rn_render uses a fixed key.
- * Independent keys give independent renders, from which per-tile standard
+/* rn_render(scene, RN_PATHTRACED, ...) with counter-based Philox4x32-10
+ * streams under the 64-bit key (key0, key1) in place of the reference's
+ * per-pixel std::mt19937 (which rn_render keeps): the same estimator, and
+ * independent keys give independent renders, from which per-tile standard
  * errors are estimated. Generalises src/capi.cpp:44-79. */
 int rn_render_seeded(const rn_scene *scene, int width, int height, float cam_z, int samples, uint32_t key0,
                      uint32_t key1, unsigned char *rgba);

@@ -551,3 +551,134 @@ void or_compat_render(const or_compat_scene *s, int W, int H, float cam_z, int s
 	}
 	or_compat_bvh_free((void *)c.bvh);
 }
+
+/* ------------------------------------- reference estimator, mt19937 -- */
+
+static float mt_jitter(mt19937 *g) { /* uniform_real_distribution<float>(-0.5, 0.5) */
+	return mt_unif(g) * 1.0f + -0.5f;
+}
+
+static v3 cosine_hemisphere_mt(v3 n, mt19937 *g) {
+	float u1 = mt_unif(g);
+	float u2 = mt_unif(g);
+	return cos_hemi(n, u1, u2);
+}
+
+/* pathTrace, src/Renderer.cpp:820-907, as written: recursion, colours
+ * combined as direct + albedo * (what the bounce returns). */
+static v3 path_trace_mt(const cctx *c, v3 o, v3 d, int depth, mt19937 *g) {
+	const or_compat_scene *s = c->s;
+	float oa[3] = {o.x, o.y, o.z}, da[3] = {d.x, d.y, d.z};
+	float t, hu, hv;
+	int ti = or_compat_bvh_intersect(c->bvh, oa, da, -1, &t, &hu, &hv);
+	if (ti < 0)
+		return divs(environment(d), 255.0f);
+	v3 point = add(o, muls(d, t));
+	int mat = s->material[ti];
+	v3 N = ld(s->normals + 3 * ti);
+	v3 col = mk((float)s->colour[3 * ti], (float)s->colour[3 * ti + 1], (float)s->colour[3 * ti + 2]);
+	if (mat == M_EMISSIVE)
+		return muls(divs(col, 255.0f), 4.0f);
+	if (mat == M_MIRROR && depth > 0) {
+		v3 rr = reflect(d, face_viewer(N, d));
+		return path_trace_mt(c, add(point, muls(rr, 1e-4f)), rr, depth - 1, g);
+	}
+	if ((mat == M_GLASS || mat == M_DISPERSIVE) && depth > 0) {
+		float ior = 1.5f;
+		v3 n = N;
+		float cosi = dot(d, n);
+		float etai = 1.0f, etat = ior;
+		if (cosi < 0.0f) {
+			cosi = -cosi;
+		} else {
+			float tmp = etai;
+			etai = etat;
+			etat = tmp;
+			n = muls(n, -1.0f);
+		}
+		float r0 = (etai - etat) / (etai + etat);
+		r0 *= r0;
+		float fres = r0 + (1.0f - r0) * powf(1.0f - cosi, 5.0f);
+		v3 refr = refract(d, n, etai / etat);
+		v3 nd = (length(refr) < 1e-6f || mt_unif(g) < fres) ? reflect(d, n) : refr;
+		return path_trace_mt(c, add(point, muls(nd, 1e-4f)), nd, depth - 1, g);
+	}
+	if (mat == M_METAL && depth > 0) {
+		v3 n = face_viewer(N, d);
+		v3 refl = reflect(d, n);
+		v3 diff = cosine_hemisphere_mt(n, g);
+		float rough = 0.2f; /* ModelTriangle::roughness default, framework/ModelTriangle.h:31 */
+		v3 nd = normalize(mixv(refl, diff, rough * rough));
+		if (dot(nd, n) <= 0.0f)
+			nd = refl;
+		v3 albedo = divs(base_colour(s, ti, hu, hv, point, muls(d, -1.0f)), 255.0f);
+		return mul(albedo, path_trace_mt(c, add(point, muls(nd, 1e-4f)), nd, depth - 1, g));
+	}
+	if (mat == M_SUBSURFACE) {
+		v3 n = face_viewer(N, d);
+		v3 albedo = divs(base_colour(s, ti, hu, hv, point, muls(d, -1.0f)), 255.0f);
+		v3 to = sub(ld(kLightPos), point);
+		float dist = length(to);
+		v3 ldir = divs(to, dist);
+		float att = kLightIntensity / (0.0f + 0.0f * dist + kLightKq * dist * dist);
+		float wd = fmaxf(0.0f, (dot(n, ldir) + 0.5f) / (1.0f + 0.5f));
+		v3 shade = add(mk(0, 0, 0), muls(mul(albedo, mk(1, 1, 1)), visibility(c, point, ti) * att * wd));
+		v3 eo = sub(point, muls(n, 1e-3f));
+		float eoa[3] = {eo.x, eo.y, eo.z}, lda[3] = {ldir.x, ldir.y, ldir.z};
+		float et, eu, ev;
+		int eh = or_compat_bvh_intersect(c->bvh, eoa, lda, ti, &et, &eu, &ev);
+		float thick = eh >= 0 ? et : 0.0f;
+		shade = add(shade, muls(mul(albedo, mk(1, 1, 1)), att * expf(-3.0f * thick) * 0.6f));
+		v3 direct = vmin(shade, mk(1.5f, 1.5f, 1.5f));
+		if (depth <= 0)
+			return direct;
+		v3 sb = cosine_hemisphere_mt(n, g);
+		return add(direct, mul(albedo, path_trace_mt(c, add(point, muls(sb, 1e-4f)), sb, depth - 1, g)));
+	}
+	v3 n = face_viewer(N, d);
+	v3 albedo = divs(base_colour(s, ti, hu, hv, point, muls(d, -1.0f)), 255.0f);
+	v3 direct = direct_light(c, point, n, albedo, ti);
+	if (depth <= 0)
+		return direct;
+	v3 bounce = cosine_hemisphere_mt(n, g);
+	v3 indirect = mul(albedo, path_trace_mt(c, add(point, muls(bounce, 1e-4f)), bounce, depth - 1, g));
+	return add(direct, indirect);
+}
+
+/* renderPathTraced (src/Renderer.cpp:1119-1170) with its own random
+ * streams: std::mt19937 seeded (y * W + x) * 9781 + 1 per pixel; per
+ * sample two jitters, one draw for the (zero) camera motion, then the
+ * path's draws. */
+void or_compat_render_mt(const or_compat_scene *s, int W, int H, float cam_z, int spp, int y0, int y1,
+                         unsigned char *rgba) {
+	cctx c;
+	c.s = s;
+	c.bvh = or_compat_bvh_build(s->verts, s->n);
+#pragma omp parallel for schedule(dynamic, 4)
+	for (int y = y0; y < y1; y++) {
+		mt19937 *g = (mt19937 *)malloc(sizeof(mt19937));
+		for (int x = 0; x < W; x++) {
+			mt_seed(g, (uint32_t)(y * W + x) * 9781u + 1u);
+			v3 sum = mk(0, 0, 0);
+			for (int si = 0; si < spp; si++) {
+				float jx = mt_jitter(g);
+				float jy = mt_jitter(g);
+				float o[3], d[3];
+				or_compat_primary_ray(x, y, W, H, cam_z, jx, jy, o, d);
+				(void)mt_unif(g); /* camera motion (zero here) */
+				sum = add(sum, path_trace_mt(&c, ld(o), ld(d), 4, g));
+			}
+			v3 col = muls(divs(sum, (float)spp), 255.0f);
+			int r = (int)col.x < 255 ? (int)col.x : 255;
+			int gg = (int)col.y < 255 ? (int)col.y : 255;
+			int b = (int)col.z < 255 ? (int)col.z : 255;
+			unsigned char *px = rgba + 4 * ((size_t)y * (size_t)W + (size_t)x);
+			px[0] = (unsigned char)(r & 0xFF);
+			px[1] = (unsigned char)(gg & 0xFF);
+			px[2] = (unsigned char)(b & 0xFF);
+			px[3] = 255;
+		}
+		free(g);
+	}
+	or_compat_bvh_free((void *)c.bvh);
+}

@@ -219,3 +219,47 @@ static v3 environment(v3 d) {
 	return mixv(mk(200.0f, 210.0f, 235.0f), mk(70.0f, 120.0f, 205.0f), t);
 }
 
+/* ---------------------------------------------------- std::mt19937 -- */
+
+/* MT19937 (Matsumoto and Nishimura, ACM TOMACS 8(1), 1998) with the seeding
+ * of std::mersenne_twister_engine ([rand.eng.mers]). */
+typedef struct {
+	uint32_t x[624];
+	int i;
+} mt19937;
+
+static void mt_seed(mt19937 *g, uint32_t s) {
+	g->x[0] = s;
+	for (int i = 1; i < 624; i++)
+		g->x[i] = 1812433253u * (g->x[i - 1] ^ (g->x[i - 1] >> 30)) + (uint32_t)i;
+	g->i = 624;
+}
+
+static uint32_t mt_next(mt19937 *g) {
+	if (g->i >= 624) {
+		for (int k = 0; k < 624; k++) {
+			uint32_t y = (g->x[k] & 0x80000000u) | (g->x[(k + 1) % 624] & 0x7FFFFFFFu);
+			uint32_t v = g->x[(k + 397) % 624] ^ (y >> 1);
+			if (y & 1u)
+				v ^= 0x9908B0DFu;
+			g->x[k] = v;
+		}
+		g->i = 0;
+	}
+	uint32_t y = g->x[g->i++];
+	y ^= y >> 11;
+	y ^= (y << 7) & 0x9D2C5680u;
+	y ^= (y << 15) & 0xEFC60000u;
+	y ^= y >> 18;
+	return y;
+}
+
+/* libstdc++ generate_canonical<float, 24> over a 32-bit engine
+ * (bits/random.tcc): one draw, float(x) / 2^32, and the largest float
+ * below 1 when the rounding reaches 1. */
+static float mt_unif(mt19937 *g) {
+	float r = (float)mt_next(g) / 4294967296.0f;
+	if (r >= 1.0f)
+		r = nextafterf(1.0f, 0.0f);
+	return r;
+}

@@ -91,6 +91,12 @@ typedef struct {
 
 void or_compat_render(const or_compat_scene *scene, int W, int H, float cam_z, int spp, uint32_t key0,
                       uint32_t key1, int y0, int y1, unsigned char *rgba);
+/* The same estimator with the reference's own per-pixel std::mt19937 streams
+ * and pathTrace's recursive colour sums: the reference's image.
+ * [pinned: pixel for pixel against renderPathTraced of the compiled
+ *  reference (tests/golden/modeR_cornell.npz).] */
+void or_compat_render_mt(const or_compat_scene *scene, int W, int H, float cam_z, int spp, int y0, int y1,
+                         unsigned char *rgba);
 
 /* Photon mapping and bidirectional path tracing (photon.c). `rng` selects
  * the reference's std::mt19937 streams (OR_RNG_MT; the images then equal the

@@ -82,6 +82,8 @@ def lib():
         L.or_fractal_noise.restype = C.c_float
         L.or_compat_render.argtypes = [C.POINTER(CompatScene), C.c_int, C.c_int, C.c_float, C.c_int, C.c_uint32,
                                        C.c_uint32, C.c_int, C.c_int, u8p]
+        L.or_compat_render_mt.argtypes = [C.POINTER(CompatScene), C.c_int, C.c_int, C.c_float, C.c_int, C.c_int,
+                                          C.c_int, u8p]
         L.or_compat_camera_ray.argtypes = [C.c_float, C.c_float, C.c_float, f32p, f32p]
         L.or_photon_trace.argtypes = [C.POINTER(CompatScene), C.c_int, C.c_int, C.c_uint32, C.c_uint32, f32p, f32p,
                                       f32p, C.c_int]

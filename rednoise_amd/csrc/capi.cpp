@@ -48,9 +48,6 @@ namespace {
 thread_local std::string t_error;
 thread_local rn_stats t_stats;
 
-// Philox key of rn_render's reference estimator.
-constexpr uint32_t kCompatKey0 = 0x243F6A88u, kCompatKey1 = 0x85A308D3u;
-
 void set_error(const std::string &m) { t_error = m; }
 
 // Process-wide device context: the library's own stream, and one lock that
@@ -553,8 +550,10 @@ int rn_scene_get_texture(const rn_scene *scene, int id, int *width, int *height,
 	return 1;
 }
 
+// RN_PATHTRACED: with mt = true the reference's own per-pixel std::mt19937
+// streams (the reference's image); otherwise Philox under the key (k0, k1).
 static int render_compat(const rn_scene *scene, int width, int height, float cam_z, int samples, uint32_t k0,
-                         uint32_t k1, unsigned char *rgba) {
+                         uint32_t k1, bool mt, unsigned char *rgba) {
 	if (samples <= 0)
 		throw Error("rn_render: samples must be >= 1 for RN_PATHTRACED");
 	if ((long long)width * height > (1ll << 29))
@@ -568,7 +567,10 @@ static int render_compat(const rn_scene *scene, int width, int height, float cam
 	sc->ensure_bvh(8, c.stream);
 	unsigned char *d_rgba = static_cast<unsigned char *>(c.out.reserve((size_t)width * height * 4));
 	RN_HIP(hipEventRecord(c.ev0, c.stream));
-	launch_compat_render(sc->dev(), sc->compat(), width, height, cam_z, samples, k0, k1, d_rgba, c.stream);
+	if (mt)
+		launch_compat_render_mt(sc->dev(), sc->compat(), width, height, cam_z, samples, d_rgba, c.stream);
+	else
+		launch_compat_render(sc->dev(), sc->compat(), width, height, cam_z, samples, k0, k1, d_rgba, c.stream);
 	RN_HIP(hipEventRecord(c.ev1, c.stream));
 	download<unsigned char>(rgba, d_rgba, (size_t)width * height * 4, c.stream);
 	RN_HIP(hipStreamSynchronize(c.stream));
@@ -631,7 +633,7 @@ int rn_render(const rn_scene *scene, rn_render_mode mode, int width, int height,
 		set_error("rn_render: only RN_PATHTRACED and RN_RAYTRACED are implemented by this library");
 		return 0;
 	}
-	return render_compat(scene, width, height, cam_z, samples, kCompatKey0, kCompatKey1, rgba);
+	return render_compat(scene, width, height, cam_z, samples, 0u, 0u, true, rgba);
 	RN_CATCH(0)
 }
 
@@ -641,7 +643,7 @@ int rn_render_seeded(const rn_scene *scene, int width, int height, float cam_z, 
 	t_error.clear();
 	if (!scene || !rgba || width <= 0 || height <= 0)
 		throw Error("rn_render_seeded: NULL scene/buffer or non-positive size");
-	return render_compat(scene, width, height, cam_z, samples, key0, key1, rgba);
+	return render_compat(scene, width, height, cam_z, samples, key0, key1, false, rgba);
 	RN_CATCH(0)
 }
 

@@ -149,6 +149,194 @@ __global__ __launch_bounds__(kCBlock) void k_compat(DevScene sc, CompatDev cd, i
 	reinterpret_cast<uchar4 *>(rgba)[pix] = px;
 }
 
+// ------------------------------------------- the reference's own streams --
+// rn_render(RN_PATHTRACED) as the reference computes it: one std::mt19937
+// per pixel seeded (y * W + x) * 9781 + 1, read through libstdc++'s
+// uniform_real_distribution<float> (src/Renderer.cpp:1138-1160), and
+// pathTrace's colours combined as the recursion returns them
+// (direct + albedo * bounce, innermost first). The generator's 624-word state
+// lives in the lane's private (scratch) memory.
+
+// MT19937 (Matsumoto and Nishimura 1998) with std::mt19937's seeding.
+struct Mt19937 {
+	uint32_t x[624];
+	int i;
+	RN_DEV void seed(uint32_t s) {
+		x[0] = s;
+#pragma unroll 1
+		for (int k = 1; k < 624; k++)
+			x[k] = 1812433253u * (x[k - 1] ^ (x[k - 1] >> 30)) + (uint32_t)k;
+		i = 624;
+	}
+	RN_DEV static uint32_t mix(uint32_t a, uint32_t b, uint32_t c) {
+		uint32_t y = (a & 0x80000000u) | (b & 0x7FFFFFFFu);
+		return c ^ (y >> 1) ^ ((y & 1u) ? 0x9908B0DFu : 0u);
+	}
+	RN_DEV void twist() {
+#pragma unroll 1
+		for (int k = 0; k < 227; k++)
+			x[k] = mix(x[k], x[k + 1], x[k + 397]);
+#pragma unroll 1
+		for (int k = 227; k < 623; k++)
+			x[k] = mix(x[k], x[k + 1], x[k - 227]);
+		x[623] = mix(x[623], x[0], x[396]);
+		i = 0;
+	}
+	RN_DEV uint32_t next() {
+		if (i >= 624)
+			twist();
+		uint32_t y = x[i++];
+		y ^= y >> 11;
+		y ^= (y << 7) & 0x9D2C5680u;
+		y ^= (y << 15) & 0xEFC60000u;
+		y ^= y >> 18;
+		return y;
+	}
+	// generate_canonical<float, 24>: float(x) / 2^32, below 1
+	RN_DEV float unif() {
+		float r = (float)next() / 4294967296.0f;
+		return r >= 1.0f ? 0x1.fffffep-1f : r;
+	}
+};
+
+constexpr int kMtLevels = 5; // pathTrace depth 4 .. 0
+
+// pathTrace (src/Renderer.cpp:820-907) for one camera ray: walks down to the
+// end of the path recording each level's (direct, albedo) and how it
+// combines with the level below, then folds the colours back up.
+template <int LDS, int SPILL>
+RN_DEV float3 compat_path_mt(const DevScene &sc, const CompatDev &cd, float3 o, float3 d, Mt19937 &g, uint64_t *lds,
+                             int lane) {
+#pragma clang fp contract(off)
+	float3 dl[kMtLevels], al[kMtLevels];
+	int kind[kMtLevels]; // 0: pass the bounce through, 1: albedo * bounce, 2: direct + albedo * bounce
+	int nl = 0;
+	float3 res;
+	int depth = 4;
+	for (;;) {
+		Hit h = traverse<0, 1, LDS, SPILL>(sc.nodes, sc.tris, o, d, INFINITY, -1, lds, kCBlock, lane);
+		if (h.prim < 0) {
+			res = gdiv(wh_environment(d), 255.0f);
+			break;
+		}
+		const int ti = h.prim;
+		const float3 point = gadd(o, gscale(d, h.t));
+		const float4 nm = cd.cnorm[ti];
+		const float3 N = xyz(nm);
+		const int mat = __float_as_int(nm.w);
+		if (mat == CM_EMISSIVE) {
+			res = gscale(gdiv(xyz(cd.ccol[ti]), 255.0f), 4.0f);
+			break;
+		}
+		float3 nd;
+		if (mat == CM_MIRROR && depth > 0) {
+			nd = greflect(d, face_viewer(N, d));
+			kind[nl] = 0;
+		} else if ((mat == CM_GLASS || mat == CM_DISPERSIVE) && depth > 0) {
+			float3 n = N;
+			float cosi = gdot(d, n);
+			float etai = 1.0f, etat = 1.5f;
+			if (cosi < 0.0f) {
+				cosi = -cosi;
+			} else {
+				etai = 1.5f;
+				etat = 1.0f;
+				n = gscale(n, -1.0f);
+			}
+			float r0 = (etai - etat) / (etai + etat);
+			r0 *= r0;
+			float fres = r0 + (1.0f - r0) * powf(1.0f - cosi, 5.0f);
+			float eta = etai / etat;
+			float dv = gdot(n, d);
+			float k = 1.0f - eta * eta * (1.0f - dv * dv);
+			float3 refr = k >= 0.0f ? gsub(gscale(d, eta), gscale(n, eta * dv + sqrtf(k))) : f3(0.0f, 0.0f, 0.0f);
+			nd = (sqrtf(gdot(refr, refr)) < 1e-6f || g.unif() < fres) ? greflect(d, n) : refr;
+			kind[nl] = 0;
+		} else if (mat == CM_METAL && depth > 0) {
+			const float3 n = face_viewer(N, d);
+			const float3 refl = greflect(d, n);
+			const float u1 = g.unif();
+			const float u2 = g.unif();
+			const float3 diff = cos_hemi(n, u1, u2);
+			const float r2 = 0.2f * 0.2f; // ModelTriangle::roughness default (framework/ModelTriangle.h:31)
+			nd = gnormalize(gmix(refl, diff, r2));
+			if (gdot(nd, n) <= 0.0f)
+				nd = refl;
+			al[nl] = gdiv(base_colour(cd, sc.tris, h.leaf, ti, h.u, h.v, point, gscale(d, -1.0f)), 255.0f);
+			kind[nl] = 1;
+		} else {
+			const float3 n = face_viewer(N, d);
+			const float3 albedo = gdiv(base_colour(cd, sc.tris, h.leaf, ti, h.u, h.v, point, gscale(d, -1.0f)), 255.0f);
+			float3 direct;
+			if (mat == CM_SUBSURFACE) { // src/Renderer.cpp:872-897
+				const float3 to = gsub(RN_LPOS, point);
+				const float dist = sqrtf(gdot(to, to));
+				const float3 ldir = gdiv(to, dist);
+				const float att = kLIntensity / (0.0f + 0.0f * dist + kLKq * dist * dist);
+				const float wd = fmaxf(0.0f, (gdot(n, ldir) + 0.5f) / (1.0f + 0.5f));
+				float3 shade = gscale(albedo, visibility<LDS, SPILL>(sc, point, ti, lds, lane) * att * wd);
+				Hit ex = traverse<0, 1, LDS, SPILL>(sc.nodes, sc.tris, gsub(point, gscale(n, 1e-3f)), ldir, INFINITY, ti,
+				                                    lds, kCBlock, lane);
+				const float thick = ex.prim >= 0 ? ex.t : 0.0f;
+				shade = gadd(shade, gscale(albedo, att * expf(-3.0f * thick) * 0.6f));
+				direct = f3(fminf(shade.x, 1.5f), fminf(shade.y, 1.5f), fminf(shade.z, 1.5f));
+			} else { // diffuse and every material that falls through to it (src/Renderer.cpp:899-906)
+				direct = direct_light<LDS, SPILL>(sc, point, n, albedo, ti, lds, lane);
+			}
+			if (depth <= 0) {
+				res = direct;
+				break;
+			}
+			const float u1 = g.unif();
+			const float u2 = g.unif();
+			nd = cos_hemi(n, u1, u2);
+			dl[nl] = direct;
+			al[nl] = albedo;
+			kind[nl] = 2;
+		}
+		nl++;
+		o = gadd(point, gscale(nd, 1e-4f));
+		d = nd;
+		depth--;
+	}
+	for (int k = nl - 1; k >= 0; k--) {
+		if (kind[k] == 1)
+			res = gmul(al[k], res);
+		else if (kind[k] == 2)
+			res = gadd(dl[k], gmul(al[k], res));
+	}
+	return res;
+}
+
+template <int LDS, int SPILL>
+__global__ __launch_bounds__(kCBlock) void k_compat_mt(DevScene sc, CompatDev cd, int W, int H, float cam_z, int spp,
+                                                       unsigned char *rgba) {
+#pragma clang fp contract(off)
+	__shared__ uint64_t stk[LDS * kCBlock];
+	uint32_t b = xcd_block(blockIdx.x, gridDim.x);
+	int tiles_x = (W + kTileW - 1) / kTileW;
+	int x = (int)(b % tiles_x) * kTileW + (threadIdx.x % kTileW);
+	int y = (int)(b / tiles_x) * kTileH + (threadIdx.x / kTileW);
+	if (x >= W || y >= H)
+		return;
+	const uint32_t pix = (uint32_t)(y * W + x);
+	Mt19937 g;
+	g.seed(pix * 9781u + 1u);
+	float3 sum = f3(0.0f, 0.0f, 0.0f);
+	for (int s = 0; s < spp; s++) {
+		const float jx = g.unif() * 1.0f + -0.5f; // uniform_real_distribution<float>(-0.5, 0.5)
+		const float jy = g.unif() * 1.0f + -0.5f;
+		float3 o, d;
+		compat_primary_ray(x, y, W, H, cam_z, jx, jy, o, d);
+		(void)g.unif(); // the camera-motion draw (the motion is zero)
+		sum = gadd(sum, compat_path_mt<LDS, SPILL>(sc, cd, o, d, g, stk, threadIdx.x));
+	}
+	const float3 c = gscale(gdiv(sum, (float)spp), 255.0f);
+	int rr = min(255, (int)c.x), gg = min(255, (int)c.y), bb = min(255, (int)c.z);
+	reinterpret_cast<uchar4 *>(rgba)[pix] =
+	    make_uchar4((unsigned char)(rr & 0xFF), (unsigned char)(gg & 0xFF), (unsigned char)(bb & 0xFF), 255);
+}
+
 // ------------------------------------------------------------ Whitted --
 // rn_render(RN_RAYTRACED): renderRaytraced (src/Renderer.cpp:909-948) with
 // the arguments rn_render passes (src/capi.cpp:59-61): Phong shading, the
@@ -367,6 +555,16 @@ void launch_whitted_render(const DevScene &sc, const CompatDev &cd, int W, int H
 		k_whitted<kLdsDepth, kSpillDepth><<<g, kCBlock, 0, s>>>(sc, cd, W, H, cam_z, rgba);
 	else
 		k_whitted<kLdsDepth, 0><<<g, kCBlock, 0, s>>>(sc, cd, W, H, cam_z, rgba);
+	RN_HIP(hipGetLastError());
+}
+
+void launch_compat_render_mt(const DevScene &sc, const CompatDev &cd, int W, int H, float cam_z, int spp,
+                             unsigned char *rgba, hipStream_t s) {
+	unsigned g = (unsigned)(((W + kTileW - 1) / kTileW) * ((H + kTileH - 1) / kTileH));
+	if (sc.depth - 1 > kLdsDepth)
+		k_compat_mt<kLdsDepth, kSpillDepth><<<g, kCBlock, 0, s>>>(sc, cd, W, H, cam_z, spp, rgba);
+	else
+		k_compat_mt<kLdsDepth, 0><<<g, kCBlock, 0, s>>>(sc, cd, W, H, cam_z, spp, rgba);
 	RN_HIP(hipGetLastError());
 }
 

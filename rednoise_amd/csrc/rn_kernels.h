@@ -64,6 +64,9 @@ void launch_whitted_render(const DevScene &sc, const CompatDev &cd, int W, int H
                            hipStream_t s);
 
 // Reference estimator (rn_render RN_PATHTRACED): RGBA8 output.
+// rn_render(RN_PATHTRACED) with the reference's per-pixel std::mt19937 streams.
+void launch_compat_render_mt(const DevScene &sc, const CompatDev &cd, int W, int H, float cam_z, int spp,
+                             unsigned char *rgba, hipStream_t s);
 void launch_compat_render(const DevScene &sc, const CompatDev &cd, int W, int H, float cam_z, int spp, uint32_t key0,
                           uint32_t key1, unsigned char *rgba, hipStream_t s);
 

@@ -100,3 +100,17 @@ def test_photon_list_shape_and_energy(rn, oracle):
     assert np.all(pw >= 0) and np.isclose(np.linalg.norm(dr, axis=1), 1.0, atol=1e-3).all()
     first = np.isclose(pw, e).all(axis=1)
     assert first.mean() > 0.3  # the first diffuse hit carries the emitted power unchanged
+
+
+def test_oracle_path_tracer_with_mt19937_equals_reference_image(rn, oracle):
+    """renderPathTraced restated with the reference's per-pixel std::mt19937
+    streams and pathTrace's recursive sums equals the compiled reference's
+    image pixel for pixel (160x120, 64 spp)."""
+    import ctypes as C
+    g = np.load(os.path.join(ROOT, "tests", "golden", "modeR_cornell.npz"))
+    ref = g["rgba_160x120_64"]
+    H, W = ref.shape[:2]
+    cs = compat(rn, oracle, "cornell-box.obj", 0.35)
+    out = np.zeros(W * H * 4, np.uint8)
+    oracle.lib().or_compat_render_mt(C.byref(cs.s), W, H, 4.0, 64, 0, H, out)
+    assert np.array_equal(out.reshape(H, W, 4), ref)
