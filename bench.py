@@ -350,11 +350,12 @@ def reference_modeR(rn, torch):
     ours = rn.Scene.load_obj(asset, 0.35)
     ours.render("pathtraced", W, H, 4.0, spp)
     t0 = time.perf_counter()
-    ours.render("pathtraced", W, H, 4.0, spp)
+    img = ours.render("pathtraced", W, H, 4.0, spp).to_numpy()
     t_gpu = time.perf_counter() - t0
+    same = float((img.reshape(-1, 4) == buf.reshape(-1, 4)).all(axis=1).mean())
     res = {"call": f"rn_render(RN_PATHTRACED) cornell-box.obj 320x240 spp={spp} cam_z=4",
            "reference_cpu_s": round(t_ref, 4), "reference_threads": os.cpu_count(), "gpu_s": round(t_gpu, 5),
-           "speedup": round(t_ref / t_gpu, 1)}
+           "speedup": round(t_ref / t_gpu, 1), "identical_pixels": round(same, 5)}
     # the Whitted mode on the same asset, 1280x960
     W2, H2 = 1280, 960
     sc = ref.rn_scene_load_obj(asset.encode(), C.c_float(0.35))
