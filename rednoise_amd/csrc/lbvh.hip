@@ -493,6 +493,120 @@ __global__ void k_ploc_merge(int m, int id_top, const int *nn, const uint32_t *m
 	cbox2[2 * (size_t)pos + 1] = hi;
 }
 
+// The last rounds of PLOC, from kPlocTail clusters down to the root, in one
+// work-group: the same rounds as the kernels above (same windows, pairs,
+// ids and order), with block barriers in place of kernel boundaries, so the
+// tree is the same and the ~10 launches and one host round trip of each
+// small round are saved. On entry: m clusters in buffer `cur`; st[0] = m,
+// st[1] = id_top (the next internal id, counting down). On exit st[0] = 1 and
+// st[1] = -1 when every cluster has merged.
+constexpr int kPlocTailBlock = 1024;
+#ifndef RN_PLOC_TAIL
+#define RN_PLOC_TAIL 8192
+#endif
+constexpr int kPlocTail = RN_PLOC_TAIL; // at most 8 clusters per thread
+
+__global__ __launch_bounds__(kPlocTailBlock) void k_ploc_tail(int n, int cur, int *cref, float4 *cbox,
+                                                                uint32_t *ccnt, int *nn, uint32_t *mflag,
+                                                                uint32_t *kflag, int *left, int *right,
+                                                                uint32_t *count, float4 *box, int *st) {
+	__shared__ uint32_t sm[kPlocTailBlock], sk[kPlocTailBlock];
+	const int tid = threadIdx.x;
+	int m = st[0], id_top = st[1];
+	while (m > 1) {
+		const int *cr = cref + (size_t)cur * n;
+		const float4 *cb = cbox + (size_t)cur * 2 * n;
+		const uint32_t *cc = ccnt + (size_t)cur * n;
+		int *cr2 = cref + (size_t)(cur ^ 1) * n;
+		float4 *cb2 = cbox + (size_t)(cur ^ 1) * 2 * n;
+		uint32_t *cc2 = ccnt + (size_t)(cur ^ 1) * n;
+		// nearest neighbours (k_ploc_nn)
+		for (int i = tid; i < m; i += kPlocTailBlock) {
+			const float4 alo = cb[2 * (size_t)i], ahi = cb[2 * (size_t)i + 1];
+			float best = INFINITY;
+			int bj = -1;
+			const int j0 = max(0, i - kPlocR), j1 = min(m - 1, i + kPlocR);
+			for (int j = j0; j <= j1; j++) {
+				if (j == i)
+					continue;
+				float a = merged_area(alo, ahi, cb[2 * (size_t)j], cb[2 * (size_t)j + 1]);
+				if (a < best) {
+					best = a;
+					bj = j;
+				}
+			}
+			nn[i] = bj;
+		}
+		__syncthreads();
+		// flags (k_ploc_flags) over this thread's contiguous chunk, and its sums
+		const int per = (m + kPlocTailBlock - 1) / kPlocTailBlock;
+		const int b0 = min(m, tid * per), b1 = min(m, b0 + per);
+		uint32_t tm = 0, tk = 0;
+		for (int i = b0; i < b1; i++) {
+			const int j = nn[i];
+			const bool mutual = j >= 0 && nn[j] == i;
+			const uint32_t mf = (mutual && i < j) ? 1u : 0u, kf = (mutual && i > j) ? 0u : 1u;
+			mflag[i] = mf;
+			kflag[i] = kf;
+			tm += mf;
+			tk += kf;
+		}
+		sm[tid] = tm;
+		sk[tid] = tk;
+		__syncthreads();
+		for (int off = 1; off < kPlocTailBlock; off <<= 1) {
+			const uint32_t xm = tid >= off ? sm[tid - off] : 0u, xk = tid >= off ? sk[tid - off] : 0u;
+			__syncthreads();
+			sm[tid] += xm;
+			sk[tid] += xk;
+			__syncthreads();
+		}
+		const uint32_t merges = sm[kPlocTailBlock - 1], kept = sk[kPlocTailBlock - 1];
+		// merge (k_ploc_merge) with this thread's exclusive offsets
+		uint32_t mo = sm[tid] - tm, ko = sk[tid] - tk;
+		for (int i = b0; i < b1; i++) {
+			const uint32_t mf = mflag[i], kf = kflag[i];
+			if (kf) {
+				const uint32_t pos = ko;
+				float4 lo = cb[2 * (size_t)i], hi = cb[2 * (size_t)i + 1];
+				if (mf) {
+					const int j = nn[i];
+					const int id = id_top - (int)mo;
+					const float4 blo = cb[2 * (size_t)j], bhi = cb[2 * (size_t)j + 1];
+					lo = make_float4(fminf(lo.x, blo.x), fminf(lo.y, blo.y), fminf(lo.z, blo.z), 0.0f);
+					hi = make_float4(fmaxf(hi.x, bhi.x), fmaxf(hi.y, bhi.y), fmaxf(hi.z, bhi.z), 0.0f);
+					left[id] = cr[i];
+					right[id] = cr[j];
+					const uint32_t c = cc[i] + cc[j];
+					count[id] = c;
+					box[2 * (size_t)id] = lo;
+					box[2 * (size_t)id + 1] = hi;
+					cr2[pos] = id;
+					cc2[pos] = c;
+				} else {
+					cr2[pos] = cr[i];
+					cc2[pos] = cc[i];
+				}
+				cb2[2 * (size_t)pos] = lo;
+				cb2[2 * (size_t)pos + 1] = hi;
+			}
+			mo += mf;
+			ko += kf;
+		}
+		__syncthreads();
+		if (merges == 0u || (int)kept != m - (int)merges)
+			break; // cannot happen (the least pair of a round is mutual); the host check reports it
+		id_top -= (int)merges;
+		m = (int)kept;
+		cur ^= 1;
+	}
+	if (tid == 0) {
+		st[0] = m;
+		st[1] = id_top;
+		st[2] = cur;
+	}
+}
+
 // -------------------------------------------------------------- collapse --
 
 struct Entry {
@@ -736,6 +850,21 @@ void ploc_tree(Bvh &b, const float *d_verts, int n, hipStream_t s) {
 	int m = n, cur = 0, id_top = n - 2;
 	uint32_t h[2];
 	while (m > 1) {
+		if (m <= kPlocTail) { // the rest in one work-group
+			int *st = reinterpret_cast<int *>(tot);
+			int hs[3] = {m, id_top, 0};
+			RN_HIP(hipMemcpyAsync(st, hs, 2 * sizeof(int), hipMemcpyHostToDevice, s));
+			k_ploc_tail<<<1, kPlocTailBlock, 0, s>>>(n, cur, cref, cbox, ccnt, nn, mflag, kflag, left, right, count,
+			                                         box, st);
+			RN_HIP(hipGetLastError());
+			RN_HIP(hipMemcpyAsync(hs, st, 3 * sizeof(int), hipMemcpyDeviceToHost, s));
+			RN_HIP(hipStreamSynchronize(s));
+			if (hs[0] != 1)
+				throw Error("PLOC: a clustering round merged nothing (internal error)");
+			m = hs[0];
+			id_top = hs[1];
+			break;
+		}
 		int *cr = cref + (size_t)cur * n, *cr2 = cref + (size_t)(cur ^ 1) * n;
 		float4 *cb = cbox + (size_t)cur * 2 * n, *cb2 = cbox + (size_t)(cur ^ 1) * 2 * n;
 		uint32_t *cc = ccnt + (size_t)cur * n, *cc2 = ccnt + (size_t)(cur ^ 1) * n;
