@@ -169,15 +169,23 @@ __global__ __launch_bounds__(256) void k_generate(WQ q, int cur) {
 	wave_add_u64(&q.cnt->shard[blockIdx.x % kCounterShards].paths, npaths);
 }
 
-// Queue entries a wave takes from the chunk counter at a time, and the number
-// of idle lanes that triggers a refill.
-#ifndef RN_TRACE_CHUNK
-#define RN_TRACE_CHUNK 128
+// Queue entries a wave takes from the chunk counter at a time: about
+// 1/kChunksPerWave of a wave's share of the queue, within [kChunkMin,
+// kChunkMax] (fewer counter atomics on long queues, balance on short ones);
+// and the number of idle lanes that triggers a refill.
+#ifndef RN_CHUNKS_PER_WAVE
+#define RN_CHUNKS_PER_WAVE 16
+#endif
+#ifndef RN_CHUNK_MIN
+#define RN_CHUNK_MIN 128
+#endif
+#ifndef RN_CHUNK_MAX
+#define RN_CHUNK_MAX 1024
 #endif
 #ifndef RN_REFILL_MIN
 #define RN_REFILL_MIN 16
 #endif
-constexpr unsigned kChunk = RN_TRACE_CHUNK;
+constexpr unsigned kChunksPerWave = RN_CHUNKS_PER_WAVE, kChunkMin = RN_CHUNK_MIN, kChunkMax = RN_CHUNK_MAX;
 constexpr int kRefillMin = RN_REFILL_MIN;
 static_assert(kRefillMin >= 1 && kRefillMin <= 64, "refill threshold is a lane count");
 
@@ -197,6 +205,8 @@ RN_DEV void trace_queue(const WQ &q, int cur, uint64_t *stk) {
 	unsigned *ctrs = QUERY == 0 ? c->trace_ctr : c->shadow_ctr;
 	// chunk j = stripe + kStripes * k is the k-th chunk of stripe j % kStripes;
 	// a wave starts on its own stripe and moves on when it is exhausted
+	const unsigned waves = gridDim.x * (unsigned)(BS / 64);
+	const unsigned kChunk = min(kChunkMax, max(kChunkMin, n / (waves * kChunksPerWave)));
 	const unsigned nch = (n + kChunk - 1) / kChunk;
 	unsigned stripe = (blockIdx.x * (unsigned)(BS / 64) + (threadIdx.x >> 6)) % (unsigned)kStripes;
 	const float4 *O = QUERY == 0 ? q.o[cur] : q.so;
@@ -593,7 +603,7 @@ void wavefront_render_on(const WaveItem *items, int count, const WaveConfig &cfg
 	// pulls chunks until the queue is exhausted. Shade and generate: plain
 	// grid-stride launches.
 	unsigned grid = (unsigned)std::max(1, prop.multiProcessorCount) * (bs == 128 ? 16u : 8u);
-	const unsigned chunks = (cap + kChunk - 1) / kChunk, waves_per_block = (unsigned)bs / 64u;
+	const unsigned chunks = (cap + kChunkMin - 1) / kChunkMin, waves_per_block = (unsigned)bs / 64u;
 	grid = std::min<unsigned>(grid, (chunks + waves_per_block - 1) / waves_per_block);
 	const unsigned sgrid = std::min<unsigned>((unsigned)std::max(1, prop.multiProcessorCount) * 8u, (cap + 255) / 256);
 	const unsigned shgrid = std::min<unsigned>((unsigned)std::max(1, prop.multiProcessorCount) * (2048u / kShadeBlock),
