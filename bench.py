@@ -326,7 +326,14 @@ def cpu_baseline(sd, W, H, spp, depth, rr, eps, budget_s=12.0):
         total_t += dt
         npix *= 2
     ps.close()
+    cpu_model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            cpu_model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+    except OSError:
+        pass
     return {"value": round(total_rays / total_t / 1e6, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "cpu_model": cpu_model,
             "sample": f"{total_paths // spp} random pixels x {spp} spp of the same 1080p frame (FP32 oracle port, "
                       f"median-split BVH, OpenMP {threads} threads), {total_t:.1f}s render + {prep:.1f}s BVH build",
             "samples_per_s": round(total_paths / total_t, 1)}
@@ -353,9 +360,18 @@ def reference_modeR(rn, torch):
     img = ours.render("pathtraced", W, H, 4.0, spp).to_numpy()
     t_gpu = time.perf_counter() - t0
     same = float((img.reshape(-1, 4) == buf.reshape(-1, 4)).all(axis=1).mean())
+    # calibration of the CPU port against the reference on the same call
+    cs = O.CompatArrays.from_scene(ours)
+    port = np.zeros(W * H * 4, np.uint8)
+    t0 = time.perf_counter()
+    O.lib().or_compat_render_mt(C.byref(cs.s), W, H, 4.0, spp, 0, H, port)
+    t_port = time.perf_counter() - t0
     res = {"call": f"rn_render(RN_PATHTRACED) cornell-box.obj 320x240 spp={spp} cam_z=4",
            "reference_cpu_s": round(t_ref, 4), "reference_threads": os.cpu_count(), "gpu_s": round(t_gpu, 5),
-           "speedup": round(t_ref / t_gpu, 1), "identical_pixels": round(same, 5)}
+           "speedup": round(t_ref / t_gpu, 1), "identical_pixels": round(same, 5),
+           "cpu_port_s": round(t_port, 4), "cpu_port_over_reference_time": round(t_port / t_ref, 3),
+           "cpu_port_identical_pixels": round(float((port.reshape(-1, 4) == buf.reshape(-1, 4)).all(axis=1).mean()),
+                                              5)}
     # the Whitted mode on the same asset, 1280x960
     W2, H2 = 1280, 960
     sc = ref.rn_scene_load_obj(asset.encode(), C.c_float(0.35))
@@ -504,7 +520,9 @@ def main():
                 "avg_launch_ms": round(kms / max(1, launches), 4), "launches": launches,
                 "bytes_per_closest_ray": round(b_closest, 1), "bytes_per_shadow_ray": round(b_shadow, 1),
                 "nodes_per_closest_ray": round(nc, 3), "tris_per_closest_ray": round(tc, 3),
-                "nodes_per_shadow_ray": round(ns, 3), "tris_per_shadow_ray": round(ts, 3)}
+                "nodes_per_shadow_ray": round(ns, 3), "tris_per_shadow_ray": round(ts, 3),
+                "compulsory_gbs": round((rc * 48 if kname.startswith("k_extend") else rs * 36) / (kms * 1e-3) / 1e9,
+                                        2)}
     megakernel_kernel_ms = sum(s["render_ms"] for s in st_m)
     mk_bytes = sum(s["rays_closest"] for s in st_m) * b_closest + sum(s["rays_shadow"] for s in st_m) * b_shadow
 

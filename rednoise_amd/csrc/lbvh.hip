@@ -15,7 +15,10 @@ namespace rn {
 
 namespace {
 
-constexpr int kLeafMax = 3;
+#ifndef RN_LEAF_MAX
+#define RN_LEAF_MAX 3
+#endif
+constexpr int kLeafMax = RN_LEAF_MAX; // leaf triangles at most (the meta byte holds 1, 3 or 7)
 constexpr int kBlock = 256;
 
 // Total order on floats as unsigned ints (for atomic min/max of bounds).
