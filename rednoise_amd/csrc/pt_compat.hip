@@ -341,27 +341,21 @@ __global__ __launch_bounds__(kCBlock) void k_compat_mt(DevScene sc, CompatDev cd
 // rn_render(RN_RAYTRACED): renderRaytraced (src/Renderer.cpp:909-948) with
 // the arguments rn_render passes (src/capi.cpp:59-61): Phong shading, the
 // default soft point light, maxDepth 4, no fog. traceRay's recursion
-// (src/Renderer.cpp:735-772) is a depth-first walk over an explicit stack of
-// weighted rays: every branch (mirror, the two glass rays weighted by
-// Fresnel, the three dispersive channels, metal tinted by its colour) is
-// linear in the colours returned below it, so the pixel is the sum of
-// weight x leaf colour. The leaf colours keep glm's FP32 operation order.
-
-struct WhRay {
-	float3 o, d, w;
-	int depth;
-};
-constexpr int kWhStack = 24; // depth 4 x (6 dispersive children - 1) + 1
+// (src/Renderer.cpp:704-772) runs on an explicit stack of open calls (mirror,
+// metal, glass, dispersive); each combines its children's colours in the
+// reference's order and FP32 operations once they have returned (the
+// reflected ray of glass before the refracted one, the three dispersive
+// channels in turn), so the pixel is the reference's sum, not a reordering.
 
 // src/Renderer.cpp:574-581
 RN_DEV float3 bump_normal(float3 n, float3 p) {
 #pragma clang fp contract(off)
-	float3 q = p * 6.0f;
+	float3 q = gscale(p, 6.0f);
 	const float e = 0.15f;
 	float3 g = f3(perlin(q.x + e, q.y, q.z) - perlin(q.x - e, q.y, q.z),
 	              perlin(q.x, q.y + e, q.z) - perlin(q.x, q.y - e, q.z),
 	              perlin(q.x, q.y, q.z + e) - perlin(q.x, q.y, q.z - e));
-	return gnormalize(n + g * 2.0f);
+	return gnormalize(gadd(n, gscale(g, 2.0f)));
 }
 
 // src/Renderer.cpp:618-639
@@ -369,15 +363,15 @@ RN_DEV float3 tangent_normal(const CompatDev &cd, const float4 *tris, uint32_t l
                              float v, float3 gn) {
 #pragma clang fp contract(off)
 	float3 v0 = xyz(tris[3 * leaf]), v1 = xyz(tris[3 * leaf + 1]), v2 = xyz(tris[3 * leaf + 2]);
-	float3 e1 = v1 - v0, e2 = v2 - v0;
+	float3 e1 = gsub(v1, v0), e2 = gsub(v2, v0);
 	const float *tp = cd.ctc + 6 * (size_t)prim;
 	float d1x = tp[2] - tp[0], d1y = tp[3] - tp[1], d2x = tp[4] - tp[0], d2y = tp[5] - tp[1];
 	float det = d1x * d2y - d2x * d1y;
 	if (fabsf(det) < 1e-8f)
 		return gn;
 	float f = 1.0f / det;
-	float3 T = gnormalize((e1 * d2y - e2 * d1y) * f);
-	float3 B = gnormalize((e1 * -d2x + e2 * d1x) * f);
+	float3 T = gnormalize(gscale(gsub(gscale(e1, d2y), gscale(e2, d1y)), f));
+	float3 B = gnormalize(gscale(gadd(gscale(e1, -d2x), gscale(e2, d1x)), f));
 	int4 ti = cd.tex_info[tex];
 	float e = 1.0f / (float)ti.y;
 	auto lum = [&](float uu, float vv) {
@@ -387,7 +381,7 @@ RN_DEV float3 tangent_normal(const CompatDev &cd, const float4 *tris, uint32_t l
 	float hu = lum(u + e, v) - lum(u - e, v);
 	float hv = lum(u, v + e) - lum(u, v - e);
 	float3 tn = gnormalize(f3(-hu * 4.0f, -hv * 4.0f, 1.0f));
-	return gnormalize(T * tn.x + B * tn.y + gn * tn.z);
+	return gnormalize(gadd(gadd(gscale(T, tn.x), gscale(B, tn.y)), gscale(gn, tn.z)));
 }
 
 // shadeSurface (src/Renderer.cpp:503-534) for the default light.
@@ -396,20 +390,20 @@ RN_DEV float3 shade_surface(const DevScene &sc, float3 point, float3 n, float3 v
                             uint64_t *lds, int lane) {
 #pragma clang fp contract(off)
 	const float ambient = 0.2f;
-	float3 to = RN_LPOS - point;
+	float3 to = gsub(RN_LPOS, point);
 	float dist = sqrtf(gdot(to, to));
-	float3 ldir = f3(to.x / dist, to.y / dist, to.z / dist);
+	float3 ldir = gdiv(to, dist);
 	float att = kLIntensity / (0.0f + 0.0f * dist + kLKq * dist * dist);
 	float vis = visibility<LDS, SPILL>(sc, point, ignore, lds, lane);
 	float inc = fmaxf(0.0f, gdot(n, ldir));
 	float bright = vis * 1.0f * att * inc;
-	float3 diff = f3(0.0f, 0.0f, 0.0f) + f3(1.0f, 1.0f, 1.0f) * bright;
+	float3 diff = gadd(f3(0.0f, 0.0f, 0.0f), gscale(f3(1.0f, 1.0f, 1.0f), bright));
 	float spec = 0.0f;
 	if (bright > 0.0f) {
-		float3 rd = n * (2.0f * gdot(n, ldir)) - ldir;
+		float3 rd = gsub(gscale(n, 2.0f * gdot(n, ldir)), ldir);
 		spec += vis * 1.0f * powf(fmaxf(0.0f, gdot(rd, view)), 64.0f);
 	}
-	float3 lit = diff + f3(ambient, ambient, ambient);
+	float3 lit = gadd(diff, f3(ambient, ambient, ambient));
 	lit = f3(fminf(1.0f, lit.x), fminf(1.0f, lit.y), fminf(1.0f, lit.z));
 	float s255 = 255.0f * fminf(1.0f, spec);
 	return f3(base.x * lit.x + s255, base.y * lit.y + s255, base.z * lit.z + s255);
@@ -422,11 +416,11 @@ RN_DEV float3 shade_diffuse(const DevScene &sc, const CompatDev &cd, const Hit &
 #pragma clang fp contract(off)
 	const int prim = h.prim;
 	float w0 = 1.0f - h.u - h.v, w1 = h.u, w2 = h.v;
-	float3 view = d * -1.0f;
+	float3 view = gscale(d, -1.0f);
 	float3 base = base_colour(cd, sc.tris, h.leaf, prim, h.u, h.v, point, view);
 	const float *vn = cd.vnorm + 9 * (size_t)prim;
 	float3 n0 = f3(vn[0], vn[1], vn[2]), n1 = f3(vn[3], vn[4], vn[5]), n2 = f3(vn[6], vn[7], vn[8]);
-	float3 n = gnormalize(n0 * w0 + n1 * w1 + n2 * w2);
+	float3 n = gnormalize(gadd(gadd(gscale(n0, w0), gscale(n1, w1)), gscale(n2, w2)));
 	n = face_viewer(n, d);
 	int mat = __float_as_int(cd.cnorm[prim].w);
 	if (mat == CM_BUMP)
@@ -440,10 +434,20 @@ RN_DEV float3 shade_diffuse(const DevScene &sc, const CompatDev &cd, const Hit &
 	return shade_surface<LDS, SPILL>(sc, point, n, view, prim, base, lds, lane);
 }
 
-// glassTrace (src/Renderer.cpp:710-731): push the reflected and refracted
-// rays weighted by Fresnel (the reflected ray alone on total internal
-// reflection).
-RN_DEV void glass_push(WhRay *st, int &sp, float3 point, float3 d, float3 N, float3 w, int depth, float ior) {
+// One open call of traceRay / glassTrace waiting for its children.
+struct WhFrame {
+	float3 point, refr, acc; // glass: refracted ray, reflected colour; metal: tint; dispersive: channels so far
+	float3 d, N;             // dispersive: the incoming ray and the face normal, for its next glassTrace
+	float fresnel;
+	int kind; // 0 mirror, 1 metal, 2 glass, 3 dispersive
+	int stage;
+	int depth;
+};
+constexpr int kWhFrames = 8; // per level at most a dispersive call and one of its glassTrace calls
+
+// glassTrace (src/Renderer.cpp:710-731) up to its first child: the frame
+// holds the refracted ray and the Fresnel weight; returns the reflected ray.
+RN_DEV float3 glass_open(WhFrame &f, float3 point, float3 d, float3 N, int depth, float ior) {
 #pragma clang fp contract(off)
 	float3 n = N;
 	float cosi = gdot(d, n);
@@ -453,78 +457,120 @@ RN_DEV void glass_push(WhRay *st, int &sp, float3 point, float3 d, float3 N, flo
 	} else {
 		etai = ior;
 		etat = 1.0f;
-		n = n * -1.0f;
+		n = gscale(n, -1.0f);
 	}
-	float3 refl = greflect(d, n);
-	float eta = etai / etat;
-	float dv = gdot(n, d);
-	float k = 1.0f - eta * eta * (1.0f - dv * dv);
-	float3 refr = k >= 0.0f ? d * eta - n * (eta * dv + sqrtf(k)) : f3(0.0f, 0.0f, 0.0f);
-	if (sqrtf(gdot(refr, refr)) < 1e-6f) {
-		if (sp < kWhStack)
-			st[sp++] = WhRay{point + refl * 1e-4f, refl, w, depth - 1};
-		return;
-	}
+	const float3 refl = greflect(d, n);
+	const float eta = etai / etat;
+	const float dv = gdot(n, d);
+	const float k = 1.0f - eta * eta * (1.0f - dv * dv);
+	f.refr = k >= 0.0f ? gsub(gscale(d, eta), gscale(n, eta * dv + sqrtf(k))) : f3(0.0f, 0.0f, 0.0f);
 	float r0 = (etai - etat) / (etai + etat);
 	r0 *= r0;
-	float fres = r0 + (1.0f - r0) * powf(1.0f - cosi, 5.0f);
-	if (sp < kWhStack)
-		st[sp++] = WhRay{point + refr * 1e-4f, refr, w * (1.0f - fres), depth - 1};
-	if (sp < kWhStack)
-		st[sp++] = WhRay{point + refl * 1e-4f, refl, w * fres, depth - 1};
+	f.fresnel = r0 + (1.0f - r0) * powf(1.0f - cosi, 5.0f);
+	f.kind = 2;
+	f.stage = 0;
+	f.point = point;
+	f.depth = depth;
+	return refl;
 }
 
 template <int LDS, int SPILL>
 RN_DEV float3 whitted_pixel(const DevScene &sc, const CompatDev &cd, float3 o, float3 d, uint64_t *lds, int lane) {
 #pragma clang fp contract(off)
-	WhRay st[kWhStack];
-	int sp = 0;
-	st[sp++] = WhRay{o, d, f3(1.0f, 1.0f, 1.0f), 4};
-	float3 col = f3(0.0f, 0.0f, 0.0f);
-	while (sp > 0) {
-		const WhRay r = st[--sp];
-		Hit h = traverse<0, 1, LDS, SPILL>(sc.nodes, sc.tris, r.o, r.d, INFINITY, -1, lds, kCBlock, lane);
+	WhFrame st[kWhFrames];
+	int sp = 0, depth = 4;
+	float3 res;
+	for (;;) {
+		// traceRay(o, d, depth) down to its first child or a leaf colour
+		Hit h = traverse<0, 1, LDS, SPILL>(sc.nodes, sc.tris, o, d, INFINITY, -1, lds, kCBlock, lane);
+		bool leaf = true;
 		if (h.prim < 0) {
-			col = col + r.w * wh_environment(r.d);
-			continue;
+			res = wh_environment(d);
+		} else {
+			const int ti = h.prim;
+			const float3 point = gadd(o, gscale(d, h.t));
+			const float4 nm = cd.cnorm[ti];
+			const float3 N = xyz(nm);
+			const int mat = __float_as_int(nm.w);
+			const float3 kd = xyz(cd.ccol[ti]);
+			if (mat == CM_EMISSIVE) {
+				res = gscale(kd, 1.3f);
+			} else if (depth > 0 && sp + 2 <= kWhFrames &&
+			           (mat == CM_MIRROR || mat == CM_GLASS || mat == CM_DISPERSIVE || mat == CM_METAL)) {
+				float3 nd;
+				if (mat == CM_MIRROR || mat == CM_METAL) {
+					WhFrame &f = st[sp++];
+					f.kind = mat == CM_MIRROR ? 0 : 1;
+					f.acc = gdiv(kd, 255.0f);
+					nd = greflect(d, face_viewer(N, d));
+				} else if (mat == CM_GLASS) {
+					nd = glass_open(st[sp++], point, d, N, depth, 1.5f);
+				} else {
+					WhFrame &f = st[sp++];
+					f.kind = 3;
+					f.stage = 0;
+					f.point = point;
+					f.d = d;
+					f.N = N;
+					f.depth = depth;
+					f.acc = f3(0.0f, 0.0f, 0.0f);
+					nd = glass_open(st[sp++], point, d, N, depth, 1.50f);
+				}
+				o = gadd(point, gscale(nd, 1e-4f));
+				d = nd;
+				depth--;
+				leaf = false;
+			} else {
+				res = shade_diffuse<LDS, SPILL>(sc, cd, h, point, d, lds, lane);
+			}
 		}
-		const int ti = h.prim;
-		const float3 point = r.o + r.d * h.t;
-		const float4 nm = cd.cnorm[ti];
-		const float3 N = xyz(nm);
-		const int mat = __float_as_int(nm.w);
-		const float3 kd = xyz(cd.ccol[ti]);
-		if (mat == CM_EMISSIVE) {
-			col = col + r.w * (kd * 1.3f);
+		if (!leaf)
 			continue;
+		// return res to the open calls
+		bool descend = false;
+		while (sp > 0 && !descend) {
+			WhFrame &f = st[sp - 1];
+			if (f.kind == 0) {
+				sp--;
+			} else if (f.kind == 1) {
+				res = gmul(f.acc, res);
+				sp--;
+			} else if (f.kind == 2) {
+				if (f.stage == 0 && sqrtf(gdot(f.refr, f.refr)) >= 1e-6f) {
+					f.acc = res;
+					f.stage = 1;
+					o = gadd(f.point, gscale(f.refr, 1e-4f));
+					d = f.refr;
+					depth = f.depth - 1;
+					descend = true;
+				} else {
+					if (f.stage == 1)
+						res = gadd(gscale(f.acc, f.fresnel), gscale(res, 1.0f - f.fresnel));
+					sp--;
+				}
+			} else { // dispersive: res is glassTrace's colour for channel f.stage
+				if (f.stage == 0)
+					f.acc.x = res.x;
+				else if (f.stage == 1)
+					f.acc.y = res.y;
+				else
+					f.acc.z = res.z;
+				if (f.stage < 2) {
+					f.stage++;
+					const float3 nd = glass_open(st[sp++], f.point, f.d, f.N, f.depth, f.stage == 1 ? 1.53f : 1.56f);
+					o = gadd(f.point, gscale(nd, 1e-4f));
+					d = nd;
+					depth = f.depth - 1;
+					descend = true;
+				} else {
+					res = f.acc;
+					sp--;
+				}
+			}
 		}
-		if (r.depth > 0 && mat == CM_MIRROR) {
-			float3 n = face_viewer(N, r.d);
-			float3 rr = greflect(r.d, n);
-			if (sp < kWhStack)
-				st[sp++] = WhRay{point + rr * 1e-4f, rr, r.w, r.depth - 1};
-			continue;
-		}
-		if (r.depth > 0 && mat == CM_GLASS) {
-			glass_push(st, sp, point, r.d, N, r.w, r.depth, 1.5f);
-			continue;
-		}
-		if (r.depth > 0 && mat == CM_DISPERSIVE) {
-			glass_push(st, sp, point, r.d, N, r.w * f3(1.0f, 0.0f, 0.0f), r.depth, 1.50f);
-			glass_push(st, sp, point, r.d, N, r.w * f3(0.0f, 1.0f, 0.0f), r.depth, 1.53f);
-			glass_push(st, sp, point, r.d, N, r.w * f3(0.0f, 0.0f, 1.0f), r.depth, 1.56f);
-			continue;
-		}
-		if (r.depth > 0 && mat == CM_METAL) {
-			float3 n = face_viewer(N, r.d);
-			float3 rr = greflect(r.d, n);
-			if (sp < kWhStack)
-				st[sp++] = WhRay{point + rr * 1e-4f, rr, r.w * (kd * (1.0f / 255.0f)), r.depth - 1};
-			continue;
-		}
-		col = col + r.w * shade_diffuse<LDS, SPILL>(sc, cd, h, point, r.d, lds, lane);
+		if (!descend)
+			return res;
 	}
-	return col;
 }
 
 template <int LDS, int SPILL>

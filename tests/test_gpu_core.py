@@ -303,3 +303,31 @@ def test_mode_r_matches_oracle_same_stream(rn, oracle):
     eq = (img[..., :3].astype(int) - out[..., :3].astype(int))
     assert (np.abs(eq) <= 1).mean() > 0.95
     assert np.abs(eq).mean() < 1.0
+
+
+@pytest.mark.parametrize("builder", ["ploc", "lbvh"])
+@pytest.mark.parametrize("n,same", [(2, False), (3, False), (5, False), (17, True), (9000, False), (9000, True)])
+def test_small_and_degenerate_builds(rn, oracle, builder, n, same):
+    """Tiny scenes, and scenes whose triangles all share one centroid (every
+    Morton code equal, every PLOC candidate pair tied): the tree is valid and
+    the closest hits equal brute force."""
+    v = soup(n, seed=n)
+    if same:
+        v = v - v.mean(axis=1, keepdims=True)  # every centroid at the origin
+    sc = rn.Scene.from_arrays(v, np.full((n, 3), 0.5, np.float32))
+    sc.set_builder(builder)
+    nodes, prims = sc.export_bvh(8)
+    check_bvh(v, nodes, prims, 8)
+    rng = np.random.default_rng(n)
+    o = rng.uniform(-8, 8, size=(512, 3)).astype(np.float32)
+    tgt = v.reshape(-1, 3)[rng.integers(0, 3 * n, 512)]
+    d = tgt - o
+    d = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    prim, t, _, _ = sc.trace_rays(o, d)
+    ps = oracle.PhysScene(v, np.full((n, 3), 0.5, np.float32), None, "f32")
+    import ctypes as C
+    for i in range(len(o)):
+        tt, uu, vv = C.c_float(), C.c_float(), C.c_float()
+        bp = oracle.lib().or_phys_closest_f32(ps.h, np.ascontiguousarray(o[i]), np.ascontiguousarray(d[i]),
+                                             np.float32(np.inf), 1, C.byref(tt), C.byref(uu), C.byref(vv))
+        assert bp == prim[i] and (bp < 0 or t[i] == tt.value), (i, bp, prim[i])
