@@ -10,7 +10,10 @@ namespace rn {
 
 // Traversal stack: kLdsDepth entries per lane in LDS, kSpillDepth more in a
 // lane-private array for BVHs deeper than kLdsDepth + 1 levels.
-constexpr int kLdsDepth = 12;
+#ifndef RN_LDS_DEPTH
+#define RN_LDS_DEPTH 12
+#endif
+constexpr int kLdsDepth = RN_LDS_DEPTH;
 constexpr int kSpillDepth = 52;
 constexpr int kMaxBvhDepth = kLdsDepth + kSpillDepth + 1;
 
