@@ -326,6 +326,7 @@ struct Trav {
 	WRay wr;
 	uint64_t grp;
 	uint32_t tri_base, thits;
+	uint32_t tri_base2, thits2; // a second pending leaf group (kTriSlots 2)
 	int sp;
 	Hit h;
 	uint64_t spill[SPILL > 0 ? SPILL : 1];
@@ -345,6 +346,8 @@ RN_DEV void trav_init(Trav<SPILL> &s, float3 o, float3 d, float tmax) {
 	s.grp = mk_group(0u, 1u, 1u << (0u ^ s.r.oct));
 	s.tri_base = 0;
 	s.thits = 0;
+	s.tri_base2 = 0;
+	s.thits2 = 0;
 }
 
 // Triangles a lane tests per step (0 = all of the node's hit leaves at
@@ -359,11 +362,20 @@ RN_DEV void trav_init(Trav<SPILL> &s, float3 o, float3 d, float tmax) {
 #endif
 constexpr int kTriBudget = RN_TRI_BUDGET;
 
+// Pending leaf groups a lane holds (1 or 2). With 2, a lane with triangles
+// still to test may visit its next node in the same step as long as the
+// second slot is free, so node visits and triangle tests overlap.
+#ifndef RN_TRI_SLOTS
+#define RN_TRI_SLOTS 2
+#endif
+constexpr int kTriSlots = RN_TRI_SLOTS;
+static_assert(kTriSlots == 1 || (kTriSlots == 2 && kTriBudget > 0), "two slots need a triangle budget");
+
 template <int QUERY, int TRI, int LDS_DEPTH, int SPILL, bool COUNT = false, bool NG = false>
 RN_DEV bool trav_step(Trav<SPILL> &s, const Bvh8Node *__restrict__ nodes, const float4 *__restrict__ tris, int ignore,
                       uint64_t *lds, int ldsStride, int lane, uint32_t *cnt = nullptr) {
 	uint32_t hits = (uint32_t)(s.grp >> 40);
-	if (hits && (kTriBudget == 0 || s.thits == 0)) {
+	if (hits && (kTriBudget == 0 || (kTriSlots == 1 ? s.thits == 0 : s.thits2 == 0))) {
 		uint32_t b = __builtin_ctz(hits);
 		hits &= hits - 1u;
 		uint32_t slot = b ^ s.r.oct;
@@ -383,10 +395,16 @@ RN_DEV bool trav_step(Trav<SPILL> &s, const Bvh8Node *__restrict__ nodes, const 
 		load_node_g(nodes, node, w);
 		if (COUNT)
 			cnt[0]++;
-		uint32_t ih;
-		intersect_children(w, s.r, s.h.t, ih, s.thits);
+		uint32_t ih, th;
+		intersect_children(w, s.r, s.h.t, ih, th);
 		s.grp = mk_group(w[4], w[3] >> 24, ih);
-		s.tri_base = w[5];
+		if (kTriSlots == 1 || s.thits == 0) {
+			s.thits = th;
+			s.tri_base = w[5];
+		} else {
+			s.thits2 = th;
+			s.tri_base2 = w[5];
+		}
 	}
 	if (s.thits) {
 		uint32_t m = s.thits;
@@ -424,9 +442,16 @@ RN_DEV bool trav_step(Trav<SPILL> &s, const Bvh8Node *__restrict__ nodes, const 
 			}
 		}
 		s.thits = m;
+		if (kTriSlots == 2 && m == 0) {
+			s.thits = s.thits2;
+			s.tri_base = s.tri_base2;
+			s.thits2 = 0;
+		}
 	}
-	if ((s.grp >> 40) == 0 && (kTriBudget == 0 || s.thits == 0)) {
+	if ((s.grp >> 40) == 0 && (kTriBudget == 0 || kTriSlots == 2 || s.thits == 0)) {
 		if (s.sp == 0) {
+			if (kTriSlots == 2 && s.thits != 0)
+				return false; // triangles left: finish them first
 			if (QUERY == 1)
 				s.h.prim = 0;
 			return true;
