@@ -463,7 +463,23 @@ rn_scene *rn_scene_load(const char *path) {
 	RN_CATCH(nullptr)
 }
 
-void rn_scene_free(rn_scene *scene) { delete scene; }
+void rn_scene_free(rn_scene *scene) {
+	if (!scene)
+		return;
+	if (scene->uploaded || scene->bvh_valid) {
+		// device work queued by an asynchronous entry point may still read
+		// the scene's buffers: drain the library stream first
+		try {
+			Context &c = ctx();
+			std::lock_guard<std::mutex> lk(c.mu);
+			(void)hipStreamSynchronize(c.stream);
+			delete scene;
+			return;
+		} catch (...) {
+		}
+	}
+	delete scene;
+}
 
 int rn_scene_triangle_count(const rn_scene *scene) { return scene ? scene->h.n : 0; }
 
