@@ -101,11 +101,17 @@ RN_DEV h16x2 h16_pair(uint32_t w, uint32_t sel) {
 }
 
 // Bit b of the result is bit (b ^ oct) of x (x has 8 significant bits).
+// fill_oct_table writes the same map as a 2 KB table, entry (oct << 8) | x.
 RN_DEV uint32_t permute_oct(uint32_t x, uint32_t oct) {
 	x = (oct & 1u) ? (((x & 0x55u) << 1) | ((x >> 1) & 0x55u)) : x;
 	x = (oct & 2u) ? (((x & 0x33u) << 2) | ((x >> 2) & 0x33u)) : x;
 	x = (oct & 4u) ? (((x & 0x0Fu) << 4) | (x >> 4)) : x;
 	return x;
+}
+
+RN_DEV void fill_oct_table(uint8_t *tab, int tid, int nthreads) {
+	for (int i = tid; i < 2048; i += nthreads)
+		tab[i] = (uint8_t)permute_oct((uint32_t)i & 0xFFu, (uint32_t)i >> 8);
 }
 
 // Intersect the (up to) 8 child boxes of a node. Returns the internal-child hit
@@ -115,7 +121,8 @@ RN_DEV uint32_t permute_oct(uint32_t x, uint32_t oct) {
 // triangle count as a unary mask (1, 3 or 7) in bits 5-7 and its offset from
 // tri_base in bits 0-4, so its triangle bits are the mask shifted by the
 // offset; internal and empty slots have meta 0.
-RN_DEV void intersect_children(const uint32_t w[20], const TRay &r, float tmax, uint32_t &ihits, uint32_t &thits) {
+RN_DEV void intersect_children(const uint32_t w[20], const TRay &r, float tmax, uint32_t &ihits, uint32_t &thits,
+                               const uint8_t *octab = nullptr) {
 	float px = __uint_as_float(w[0]), py = __uint_as_float(w[1]), pz = __uint_as_float(w[2]);
 	uint32_t e = w[3];
 	// Plane bytes are widened to f16 denormals q * 2^-24 with one v_perm per
@@ -165,7 +172,7 @@ RN_DEV void intersect_children(const uint32_t w[20], const TRay &r, float tmax, 
 			acc |= hit ? bits : 0u;
 		}
 	}
-	ihits = permute_oct((acc >> 24) & imask, r.oct);
+	ihits = octab ? (uint32_t)octab[(r.oct << 8) | ((acc >> 24) & imask)] : permute_oct((acc >> 24) & imask, r.oct);
 	thits = acc & 0xFFFFFFu;
 }
 
@@ -373,7 +380,7 @@ static_assert(kTriSlots == 1 || (kTriSlots == 2 && kTriBudget > 0), "two slots n
 
 template <int QUERY, int TRI, int LDS_DEPTH, int SPILL, bool COUNT = false, bool NG = false>
 RN_DEV bool trav_step(Trav<SPILL> &s, const Bvh8Node *__restrict__ nodes, const float4 *__restrict__ tris, int ignore,
-                      uint64_t *lds, int ldsStride, int lane, uint32_t *cnt = nullptr) {
+                      uint64_t *lds, int ldsStride, int lane, uint32_t *cnt = nullptr, const uint8_t *octab = nullptr) {
 	uint32_t hits = (uint32_t)(s.grp >> 40);
 	if (hits && (kTriBudget == 0 || (kTriSlots == 1 ? s.thits == 0 : s.thits2 == 0))) {
 		uint32_t b = __builtin_ctz(hits);
@@ -396,7 +403,7 @@ RN_DEV bool trav_step(Trav<SPILL> &s, const Bvh8Node *__restrict__ nodes, const 
 		if (COUNT)
 			cnt[0]++;
 		uint32_t ih, th;
-		intersect_children(w, s.r, s.h.t, ih, th);
+		intersect_children(w, s.r, s.h.t, ih, th, octab);
 		s.grp = mk_group(w[4], w[3] >> 24, ih);
 		if (kTriSlots == 1 || s.thits == 0) {
 			s.thits = th;

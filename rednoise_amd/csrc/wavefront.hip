@@ -199,7 +199,7 @@ static_assert(kRefillMin >= 1 && kRefillMin <= 64, "refill threshold is a lane c
 // all 64 lanes reach each __ballot/__shfl together. The loop ends once the
 // chunk counter has passed the queue length and every lane is idle.
 template <int QUERY, int LDS, int SPILL, int BS, bool ONE>
-RN_DEV void trace_queue(const WQ &q, int cur, uint64_t *stk) {
+RN_DEV void trace_queue(const WQ &q, int cur, uint64_t *stk, const uint8_t *octab) {
 	WCtl *c = q.ctl;
 	const unsigned n = QUERY == 0 ? c->cur_count + c->gen_count : c->sh_count;
 	unsigned *ctrs = QUERY == 0 ? c->trace_ctr : c->shadow_ctr;
@@ -277,7 +277,8 @@ RN_DEV void trace_queue(const WQ &q, int cur, uint64_t *stk) {
 				break;
 			continue;
 		}
-		if (valid && trav_step<QUERY, 0, LDS, SPILL, false, QUERY == 0>(t, nodes, tris, -1, stk, BS, threadIdx.x)) {
+		if (valid && trav_step<QUERY, 0, LDS, SPILL, false, QUERY == 0>(t, nodes, tris, -1, stk, BS, threadIdx.x, nullptr,
+		                                                                  octab)) {
 			valid = false;
 			if (QUERY == 0) {
 				q.hitp[idx] = t.h.prim;
@@ -307,16 +308,32 @@ RN_DEV void trace_queue(const WQ &q, int cur, uint64_t *stk) {
 #define RN_TRACE_ATTR __attribute__((amdgpu_waves_per_eu((SPILL == 0 && ONE) ? RN_TRACE_WPE : 0, \
                                                          (SPILL == 0 && ONE) ? RN_TRACE_WPE : 0)))
 
+// The octant permutation of a node's hit mask as a table in LDS (one byte
+// load per node visit instead of three conditional bit swaps).
+#ifndef RN_OCT_TABLE
+#define RN_OCT_TABLE 1
+#endif
+
 template <int LDS, int SPILL, int BS, bool ONE>
 __global__ __launch_bounds__(BS) RN_TRACE_ATTR void k_extend(WQ q, int cur) {
 	__shared__ uint64_t stk[LDS * BS];
-	trace_queue<0, LDS, SPILL, BS, ONE>(q, cur, stk);
+	__shared__ uint8_t octab[RN_OCT_TABLE ? 2048 : 1];
+	if (RN_OCT_TABLE) {
+		fill_oct_table(octab, threadIdx.x, BS);
+		__syncthreads();
+	}
+	trace_queue<0, LDS, SPILL, BS, ONE>(q, cur, stk, RN_OCT_TABLE ? octab : nullptr);
 }
 
 template <int LDS, int SPILL, int BS, bool ONE>
 __global__ __launch_bounds__(BS) RN_TRACE_ATTR void k_shadow(WQ q) {
 	__shared__ uint64_t stk[LDS * BS];
-	trace_queue<1, LDS, SPILL, BS, ONE>(q, 0, stk);
+	__shared__ uint8_t octab[RN_OCT_TABLE ? 2048 : 1];
+	if (RN_OCT_TABLE) {
+		fill_oct_table(octab, threadIdx.x, BS);
+		__syncthreads();
+	}
+	trace_queue<1, LDS, SPILL, BS, ONE>(q, 0, stk, RN_OCT_TABLE ? octab : nullptr);
 }
 
 // Shade every queued path at its closest hit: emission (camera rays only),
