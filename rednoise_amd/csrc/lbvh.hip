@@ -384,7 +384,10 @@ __global__ __launch_bounds__(kRefitChunk) void k_refit(const float *v, const uin
 // Karras tree (internal ids 0..n-2 with root 0, leaves ~position, boxes at
 // 2*id and 2*(n-1+position), triangle counts), so the collapse takes either.
 
-constexpr int kPlocR = 16;
+#ifndef RN_PLOC_R
+#define RN_PLOC_R 8
+#endif
+constexpr int kPlocR = RN_PLOC_R; // PLOC search radius (clusters on each side)
 
 __global__ void k_leaf_boxes(const float *v, const uint32_t *order, int n, float4 *box, int *cref, float4 *cbox,
                              uint32_t *ccnt) {
