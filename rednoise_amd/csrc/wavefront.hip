@@ -206,8 +206,8 @@ RN_DEV void trace_queue(const WQ &q, int cur, uint64_t *stk, const uint8_t *octa
 	// chunk j = stripe + kStripes * k is the k-th chunk of stripe j % kStripes;
 	// a wave starts on its own stripe and moves on when it is exhausted
 	const unsigned waves = gridDim.x * (unsigned)(BS / 64);
-	const unsigned kChunk = min(kChunkMax, max(kChunkMin, n / (waves * kChunksPerWave)));
-	const unsigned nch = (n + kChunk - 1) / kChunk;
+	const unsigned chunk = min(kChunkMax, max(kChunkMin, n / (waves * kChunksPerWave)));
+	const unsigned nch = (n + chunk - 1) / chunk;
 	unsigned stripe = (blockIdx.x * (unsigned)(BS / 64) + (threadIdx.x >> 6)) % (unsigned)kStripes;
 	const float4 *O = QUERY == 0 ? q.o[cur] : q.so;
 	const float4 *D = QUERY == 0 ? q.d[cur] : q.sd;
@@ -235,7 +235,7 @@ RN_DEV void trace_queue(const WQ &q, int cur, uint64_t *stk, const uint8_t *octa
 							unsigned k = atomicAdd(ctr, 1u);
 							unsigned j = stripe + (unsigned)kStripes * k;
 							if (j < nch) {
-								cb = j * kChunk;
+								cb = j * chunk;
 								break;
 							}
 						}
@@ -248,7 +248,7 @@ RN_DEV void trace_queue(const WQ &q, int cur, uint64_t *stk, const uint8_t *octa
 					more = false;
 				} else {
 					next = cb;
-					end = min(cb + kChunk, n);
+					end = min(cb + chunk, n);
 				}
 			}
 			if (next < end) {
