@@ -647,6 +647,51 @@ __device__ __forceinline__ uint32_t scale_exp(float ext) {
 	return e < 1 ? 1u : e;
 }
 
+// Grid origin of one axis of a node: child planes are stored as p + q s,
+// q = 0..255, and any p <= lo with p + 255 s >= hi holds all children. A
+// child thinner than s along the axis (a wall, an emitter, an axis-aligned
+// triangle) gains up to two steps of thickness unless its plane lies on a
+// grid line; rays that start or end on such a surface then enter its box.
+// The origin chosen puts the most flat child area (the child box's face
+// across the axis) on grid lines; p = lo when there is no flat child.
+#ifndef RN_GRID_ALIGN
+#define RN_GRID_ALIGN 1
+#endif
+__device__ float grid_origin(const Entry *e, int m, int a, float lo, float hi, float s) {
+	float plo[8], area[8];
+	int nf = 0;
+	for (int j = 0; j < m; j++) {
+		const float3 d = e[j].hi - e[j].lo;
+		const float ext = a == 0 ? d.x : a == 1 ? d.y : d.z;
+		if (ext < s) {
+			plo[nf] = a == 0 ? e[j].lo.x : a == 1 ? e[j].lo.y : e[j].lo.z;
+			area[nf] = a == 0 ? d.y * d.z : a == 1 ? d.x * d.z : d.x * d.y;
+			nf++;
+		}
+	}
+	const float is = 1.0f / s; // a power of two: exact
+	auto on_grid = [&](float p) {
+		float w = 0.0f;
+		for (int k = 0; k < nf; k++) {
+			const float q = (plo[k] - p) * is;
+			w += q == floorf(q) ? area[k] : 0.0f;
+		}
+		return w;
+	};
+	float best = lo, bw = on_grid(lo);
+	for (int j = 0; j < nf; j++) {
+		float p = plo[j] - s * ceilf((plo[j] - lo) * is);
+		if (p > lo || (double)p + 255.0 * (double)s < (double)hi) // exact: every child must fit
+			continue;
+		const float w = on_grid(p);
+		if (w > bw) {
+			bw = w;
+			best = p;
+		}
+	}
+	return best;
+}
+
 __global__ void k_collapse(int k0, int k1, int width, int *task, const float4 *box, const int *left, const int *right,
                            const uint32_t *count, const uint32_t *order, const float *verts, int n, Bvh8Node *nodes,
                            float4 *tris, uint32_t *counters, int node_cap) {
@@ -739,6 +784,10 @@ __global__ void k_collapse(int k0, int k1, int width, int *task, const float4 *b
 	Bvh8Node nd;
 	uint32_t ex = scale_exp(hi.x - lo.x), ey = scale_exp(hi.y - lo.y), ez = scale_exp(hi.z - lo.z);
 	double sx = ldexp(1.0, (int)ex - 127), sy = ldexp(1.0, (int)ey - 127), sz = ldexp(1.0, (int)ez - 127);
+	if (RN_GRID_ALIGN) {
+		lo = f3(grid_origin(e, m, 0, lo.x, hi.x, (float)sx), grid_origin(e, m, 1, lo.y, hi.y, (float)sy),
+		        grid_origin(e, m, 2, lo.z, hi.z, (float)sz));
+	}
 	nd.p[0] = lo.x;
 	nd.p[1] = lo.y;
 	nd.p[2] = lo.z;
