@@ -19,6 +19,7 @@ namespace {
 #define RN_LEAF_MAX 3
 #endif
 constexpr int kLeafMax = RN_LEAF_MAX; // leaf triangles at most (the meta byte holds 1, 3 or 7)
+static_assert(kLeafMax >= 1 && kLeafMax <= 3, "a leaf's unary count byte holds at most 3 triangles");
 constexpr int kBlock = 256;
 
 // Total order on floats as unsigned ints (for atomic min/max of bounds).
@@ -654,32 +655,39 @@ __device__ __forceinline__ uint32_t scale_exp(float ext) {
 // grid line; rays that start or end on such a surface then enter its box.
 // The origin chosen puts the most flat child area (the child box's face
 // across the axis) on grid lines; p = lo when there is no flat child.
+// Entries are read at compile-time indices only, so they stay in registers.
 #ifndef RN_GRID_ALIGN
 #define RN_GRID_ALIGN 1
 #endif
-__device__ float grid_origin(const Entry *e, int m, int a, float lo, float hi, float s) {
+__device__ __forceinline__ float grid_origin(const Entry (&e)[8], int m, int a, float lo, float hi, float s) {
 	float plo[8], area[8];
-	int nf = 0;
-	for (int j = 0; j < m; j++) {
+	uint32_t flat = 0;
+#pragma unroll
+	for (int j = 0; j < 8; j++) {
 		const float3 d = e[j].hi - e[j].lo;
 		const float ext = a == 0 ? d.x : a == 1 ? d.y : d.z;
-		if (ext < s) {
-			plo[nf] = a == 0 ? e[j].lo.x : a == 1 ? e[j].lo.y : e[j].lo.z;
-			area[nf] = a == 0 ? d.y * d.z : a == 1 ? d.x * d.z : d.x * d.y;
-			nf++;
-		}
+		plo[j] = a == 0 ? e[j].lo.x : a == 1 ? e[j].lo.y : e[j].lo.z;
+		area[j] = a == 0 ? d.y * d.z : a == 1 ? d.x * d.z : d.x * d.y;
+		if (j < m && ext < s)
+			flat |= 1u << j;
 	}
+	if (!flat)
+		return lo;
 	const float is = 1.0f / s; // a power of two: exact
 	auto on_grid = [&](float p) {
 		float w = 0.0f;
-		for (int k = 0; k < nf; k++) {
+#pragma unroll
+		for (int k = 0; k < 8; k++) {
 			const float q = (plo[k] - p) * is;
-			w += q == floorf(q) ? area[k] : 0.0f;
+			w += ((flat >> k) & 1u) && q == floorf(q) ? area[k] : 0.0f;
 		}
 		return w;
 	};
 	float best = lo, bw = on_grid(lo);
-	for (int j = 0; j < nf; j++) {
+#pragma unroll
+	for (int j = 0; j < 8; j++) {
+		if (!((flat >> j) & 1u))
+			continue;
 		float p = plo[j] - s * ceilf((plo[j] - lo) * is);
 		if (p > lo || (double)p + 255.0 * (double)s < (double)hi) // exact: every child must fit
 			continue;
@@ -692,7 +700,14 @@ __device__ float grid_origin(const Entry *e, int m, int a, float lo, float hi, f
 	return best;
 }
 
-__global__ void k_collapse(int k0, int k1, int width, int *task, const float4 *box, const int *left, const int *right,
+// One BVH8 node per thread, one tree level per launch. The entry list, the
+// slot scores and the node record are only indexed at compile-time indices
+// (loops unrolled over the 8 slots with predicates), so they live in
+// registers rather than in per-thread scratch memory.
+constexpr int kCollapseBlock = 64;
+
+__global__ __launch_bounds__(kCollapseBlock) void k_collapse(int k0, int k1, int width, int *task, const float4 *box,
+                                                             const int *left, const int *right,
                            const uint32_t *count, const uint32_t *order, const float *verts, int n, Bvh8Node *nodes,
                            float4 *tris, uint32_t *counters, int node_cap) {
 	int k = k0 + blockIdx.x * blockDim.x + threadIdx.x;
@@ -712,64 +727,85 @@ __global__ void k_collapse(int k0, int k1, int width, int *task, const float4 *b
 		// entries first (fewer levels), then small ones (tighter leaf boxes
 		// at no extra node visit).
 		while (m < width) {
-			int best = -1;
+			int best = -1, id = 0;
 			float ba = -1.0f;
-			for (int j = 0; j < m; j++) {
-				if (e[j].id >= 0) {
+#pragma unroll
+			for (int j = 0; j < 8; j++) {
+				if (j < m && e[j].id >= 0) {
 					float a = half_area(e[j]);
 					if (a > ba) {
 						ba = a;
 						best = j;
+						id = e[j].id;
 					}
 				}
 			}
 			if (best < 0)
 				break;
-			int id = e[best].id;
-			e[best] = make_entry(left[id], n, box, count);
-			e[m++] = make_entry(right[id], n, box, count);
+			const Entry l = make_entry(left[id], n, box, count), r = make_entry(right[id], n, box, count);
+#pragma unroll
+			for (int j = 0; j < 8; j++) {
+				if (j == best)
+					e[j] = l;
+				else if (j == m)
+					e[j] = r;
+			}
+			m++;
 		}
 	}
 	float3 lo = e[0].lo, hi = e[0].hi;
-	for (int j = 1; j < m; j++) {
-		lo = f3(fminf(lo.x, e[j].lo.x), fminf(lo.y, e[j].lo.y), fminf(lo.z, e[j].lo.z));
-		hi = f3(fmaxf(hi.x, e[j].hi.x), fmaxf(hi.y, e[j].hi.y), fmaxf(hi.z, e[j].hi.z));
+#pragma unroll
+	for (int j = 1; j < 8; j++) {
+		if (j < m) {
+			lo = f3(fminf(lo.x, e[j].lo.x), fminf(lo.y, e[j].lo.y), fminf(lo.z, e[j].lo.z));
+			hi = f3(fmaxf(hi.x, e[j].hi.x), fmaxf(hi.y, e[j].hi.y), fmaxf(hi.z, e[j].hi.z));
+		}
 	}
 	// Octant slot assignment: greedily give each (entry, slot) pair with the
-	// largest projection of the entry centre onto the slot diagonal.
+	// largest projection of the entry centre onto the slot diagonal (the
+	// first strict maximum in entry-major order each round).
 	float3 c = 0.5f * (lo + hi);
-	int slot_entry[8];
-	for (int s = 0; s < 8; s++)
-		slot_entry[s] = -1;
-	uint32_t used = 0;
+	float score[8][8];
+#pragma unroll
+	for (int j = 0; j < 8; j++) {
+		float3 dc = 0.5f * (e[j].lo + e[j].hi) - c;
+#pragma unroll
+		for (int s = 0; s < 8; s++)
+			score[j][s] = ((s & 1) ? dc.x : -dc.x) + ((s & 2) ? dc.y : -dc.y) + ((s & 4) ? dc.z : -dc.z);
+	}
+	uint32_t slot_of = 0; // 4 bits per entry: its slot
+	uint32_t used = 0, taken = 0;
 	for (int it = 0; it < m; it++) {
 		float bs = -INFINITY;
-		int bj = -1, bsl = -1;
-		for (int j = 0; j < m; j++) {
-			if (used & (1u << j))
-				continue;
-			float3 dc = 0.5f * (e[j].lo + e[j].hi) - c;
+		int bj = 0, bsl = 0;
+#pragma unroll
+		for (int j = 0; j < 8; j++) {
+#pragma unroll
 			for (int s = 0; s < 8; s++) {
-				if (slot_entry[s] >= 0)
-					continue;
-				float sc = ((s & 1) ? dc.x : -dc.x) + ((s & 2) ? dc.y : -dc.y) + ((s & 4) ? dc.z : -dc.z);
-				if (sc > bs) {
-					bs = sc;
+				if (j < m && !((used >> j) & 1u) && !((taken >> s) & 1u) && score[j][s] > bs) {
+					bs = score[j][s];
 					bj = j;
 					bsl = s;
 				}
 			}
 		}
+		if (bs == -INFINITY) { // no ordered score (non-finite boxes): first free entry and slot
+			bj = __builtin_ctz(~used);
+			bsl = __builtin_ctz(~taken);
+		}
 		used |= 1u << bj;
-		slot_entry[bsl] = bj;
+		taken |= 1u << bsl;
+		slot_of |= (uint32_t)bsl << (4 * bj);
 	}
 	int n_int = 0, n_tri = 0;
-	for (int j = 0; j < m; j++) {
-		int cnt = e[j].cnt;
-		if (cnt > kLeafMax)
-			n_int++;
-		else
-			n_tri += cnt;
+#pragma unroll
+	for (int j = 0; j < 8; j++) {
+		if (j < m) {
+			if (e[j].cnt > kLeafMax)
+				n_int++;
+			else
+				n_tri += e[j].cnt;
+		}
 	}
 	uint32_t child_base = n_int ? atomicAdd(&counters[0], (uint32_t)n_int) : 0u;
 	uint32_t tri_base = n_tri ? atomicAdd(&counters[1], (uint32_t)n_tri) : 0u;
@@ -784,6 +820,8 @@ __global__ void k_collapse(int k0, int k1, int width, int *task, const float4 *b
 	Bvh8Node nd;
 	uint32_t ex = scale_exp(hi.x - lo.x), ey = scale_exp(hi.y - lo.y), ez = scale_exp(hi.z - lo.z);
 	double sx = ldexp(1.0, (int)ex - 127), sy = ldexp(1.0, (int)ey - 127), sz = ldexp(1.0, (int)ez - 127);
+	// the scales are powers of two: multiplying by 1 / s is the exact division
+	const double rx = ldexp(1.0, 127 - (int)ex), ry = ldexp(1.0, 127 - (int)ey), rz = ldexp(1.0, 127 - (int)ez);
 	if (RN_GRID_ALIGN) {
 		lo = f3(grid_origin(e, m, 0, lo.x, hi.x, (float)sx), grid_origin(e, m, 1, lo.y, hi.y, (float)sy),
 		        grid_origin(e, m, 2, lo.z, hi.z, (float)sz));
@@ -796,60 +834,93 @@ __global__ void k_collapse(int k0, int k1, int width, int *task, const float4 *b
 	nd.e[2] = (uint8_t)ez;
 	nd.child_base = child_base;
 	nd.tri_base = tri_base;
-	uint32_t imask = 0, ic = 0, off = 0;
+#pragma unroll
 	for (int s = 0; s < 8; s++) {
-		int j = slot_entry[s];
 		nd.meta[s] = 0;
-		if (j < 0) {
-			for (int a = 0; a < 3; a++) {
-				nd.qlo[a][s] = 255;
-				nd.qhi[a][s] = 0;
-			}
-			continue;
-		}
-		const Entry &q = e[j];
-		double qlo[3] = {floor(((double)q.lo.x - (double)lo.x) / sx), floor(((double)q.lo.y - (double)lo.y) / sy),
-		                 floor(((double)q.lo.z - (double)lo.z) / sz)};
-		double qhi[3] = {ceil(((double)q.hi.x - (double)lo.x) / sx), ceil(((double)q.hi.y - (double)lo.y) / sy),
-		                 ceil(((double)q.hi.z - (double)lo.z) / sz)};
 		for (int a = 0; a < 3; a++) {
-			nd.qlo[a][s] = (uint8_t)fmin(fmax(qlo[a], 0.0), 255.0);
-			nd.qhi[a][s] = (uint8_t)fmin(fmax(qhi[a], 0.0), 255.0);
+			nd.qlo[a][s] = 255;
+			nd.qhi[a][s] = 0;
 		}
-		int cnt = q.cnt;
-		if (cnt > kLeafMax) {
-			imask |= 1u << s;
-			task[child_base + ic] = q.id;
-			ic++;
-		} else {
-			nd.meta[s] = (uint8_t)((((1u << cnt) - 1u) << 5) | off);
-			// the subtree's (<= kLeafMax) sorted positions, left to right
-			int pos[kLeafMax], np = 0, stk[kLeafMax], sp = 0;
-			stk[sp++] = q.id;
-			while (sp > 0) {
-				int c = stk[--sp];
-				if (c < 0) {
-					if (np < kLeafMax)
-						pos[np++] = ~c;
-				} else if (sp + 2 <= kLeafMax) {
-					stk[sp++] = right[c];
-					stk[sp++] = left[c];
+	}
+	// entries in slot order: internal children take consecutive nodes from
+	// child_base and leaf triangles consecutive records from tri_base
+	uint32_t imask = 0, ic = 0, off = 0;
+#pragma unroll
+	for (int s = 0; s < 8; s++) {
+		int j = -1;
+#pragma unroll
+		for (int jj = 0; jj < 8; jj++)
+			if (jj < m && ((slot_of >> (4 * jj)) & 15u) == (uint32_t)s)
+				j = jj;
+		if (j >= 0) {
+			Entry q = e[0];
+#pragma unroll
+			for (int jj = 1; jj < 8; jj++)
+				if (jj == j)
+					q = e[jj];
+			double qlo[3] = {floor(((double)q.lo.x - (double)lo.x) * rx), floor(((double)q.lo.y - (double)lo.y) * ry),
+			                 floor(((double)q.lo.z - (double)lo.z) * rz)};
+			double qhi[3] = {ceil(((double)q.hi.x - (double)lo.x) * rx), ceil(((double)q.hi.y - (double)lo.y) * ry),
+			                 ceil(((double)q.hi.z - (double)lo.z) * rz)};
+			for (int a = 0; a < 3; a++) {
+				nd.qlo[a][s] = (uint8_t)fmin(fmax(qlo[a], 0.0), 255.0);
+				nd.qhi[a][s] = (uint8_t)fmin(fmax(qhi[a], 0.0), 255.0);
+			}
+			const int cnt = q.cnt;
+			if (cnt > kLeafMax) {
+				imask |= 1u << s;
+				task[child_base + ic] = q.id;
+				ic++;
+			} else {
+				nd.meta[s] = (uint8_t)((((1u << cnt) - 1u) << 5) | off);
+				// the subtree's (<= kLeafMax = 3) sorted positions, left to
+				// right: an internal child of a 3-triangle subtree has two
+				// leaf children
+				int p0 = 0, p1 = 0, p2 = 0, np = 0;
+				auto put = [&](int pos) {
+					if (np == 0)
+						p0 = pos;
+					else if (np == 1)
+						p1 = pos;
+					else
+						p2 = pos;
+					np++;
+				};
+				if (q.id < 0) {
+					put(~q.id);
+				} else {
+					const int l = left[q.id], r = right[q.id];
+					if (l < 0) {
+						put(~l);
+					} else {
+						put(~left[l]);
+						put(~right[l]);
+					}
+					if (r < 0) {
+						put(~r);
+					} else {
+						put(~left[r]);
+						put(~right[r]);
+					}
 				}
+				for (int r = 0; r < kLeafMax; r++) {
+					if (r >= cnt || r >= np)
+						break;
+					uint32_t prim = order[r == 0 ? p0 : r == 1 ? p1 : p2];
+					const float *t = verts + 9 * (size_t)prim;
+					size_t dst = 3 * (size_t)(tri_base + off + r);
+					tris[dst] = make_float4(t[0], t[1], t[2], __int_as_float((int)prim));
+					tris[dst + 1] = make_float4(t[3], t[4], t[5], 0.0f);
+					tris[dst + 2] = make_float4(t[6], t[7], t[8], 0.0f);
+				}
+				off += cnt;
 			}
-			for (int r = 0; r < cnt && r < np; r++) {
-				uint32_t prim = order[pos[r]];
-				const float *t = verts + 9 * (size_t)prim;
-				size_t dst = 3 * (size_t)(tri_base + off + r);
-				tris[dst] = make_float4(t[0], t[1], t[2], __int_as_float((int)prim));
-				tris[dst + 1] = make_float4(t[3], t[4], t[5], 0.0f);
-				tris[dst + 2] = make_float4(t[6], t[7], t[8], 0.0f);
-			}
-			off += cnt;
 		}
 	}
 	nd.imask = (uint8_t)imask;
 	const uint4 *src = reinterpret_cast<const uint4 *>(&nd);
 	uint4 *dst = reinterpret_cast<uint4 *>(nodes + k);
+#pragma unroll
 	for (int q = 0; q < 5; q++)
 		dst[q] = src[q];
 }
@@ -1044,7 +1115,7 @@ void build_bvh(Bvh &b, const float *d_verts, int n, int width, hipStream_t s, in
 		int depth = 0;
 		bool overflow = false;
 		while (lo < hi) {
-			k_collapse<<<grid_for(hi - lo, 64), 64, 0, s>>>(
+			k_collapse<<<grid_for(hi - lo, kCollapseBlock), kCollapseBlock, 0, s>>>(
 			    (int)lo, (int)hi, width, task, b.boxes.as<float4>(), b.left.as<int>(), b.right.as<int>(),
 			    b.count.as<uint32_t>(), b.vals.as<uint32_t>(), d_verts, n, nodes, tris, counters, cap);
 			RN_HIP(hipMemcpyAsync(h, counters, 3 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
