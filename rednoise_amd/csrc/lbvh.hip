@@ -419,8 +419,11 @@ __device__ __forceinline__ float merged_area(float4 alo, float4 ahi, float4 blo,
 // Nearest neighbour of every cluster. Pairs are ordered by (merged area,
 // smaller index, larger index), a strict total order, so the least pair of
 // the round is always mutual and every round merges at least once.
-__global__ __launch_bounds__(kBlock) void k_ploc_nn(int m, const float4 *cbox, int *nn) {
+__global__ __launch_bounds__(kBlock) void k_ploc_nn(const uint32_t *pm, const float4 *cbox, int *nn) {
 	__shared__ float4 slo[kBlock + 2 * kPlocR], shi[kBlock + 2 * kPlocR];
+	const int m = (int)*pm;
+	if (m <= 1 || (int)(blockIdx.x * kBlock) >= m)
+		return;
 	const int base = blockIdx.x * kBlock - kPlocR;
 	for (int k = threadIdx.x; k < kBlock + 2 * kPlocR; k += kBlock) {
 		int g = base + k;
@@ -453,28 +456,53 @@ __global__ __launch_bounds__(kBlock) void k_ploc_nn(int m, const float4 *cbox, i
 	nn[i] = bj;
 }
 
-__global__ void k_ploc_flags(int m, const int *nn, uint32_t *mflag, uint32_t *kflag) {
+// Flags of the clusters [0, m); the zeros written over [m, m_ub) keep the
+// scans over the host's bound m_ub exact.
+__global__ void k_ploc_flags(const uint32_t *pm, int m_ub, const int *nn, uint32_t *mflag, uint32_t *kflag) {
 	int i = blockIdx.x * blockDim.x + threadIdx.x;
-	if (i >= m)
+	const int m = (int)*pm;
+	if (i >= m_ub)
 		return;
+	if (i >= m || m <= 1) {
+		mflag[i] = 0u;
+		kflag[i] = 0u;
+		return;
+	}
 	int j = nn[i];
 	bool mutual = j >= 0 && nn[j] == i;
 	mflag[i] = (mutual && i < j) ? 1u : 0u;
 	kflag[i] = (mutual && i > j) ? 0u : 1u;
 }
 
-__global__ void k_ploc_totals(int m, const uint32_t *mflag, const uint32_t *moff, const uint32_t *kflag,
+// Round state on the device, so that rounds run without the host:
+// tot[0] merges and tot[1] clusters kept by the round, tot[2] set when a
+// round with m > 1 merged nothing, tot[3] m, tot[4] id_top.
+constexpr int kPlocMerges = 0, kPlocKept = 1, kPlocErr = 2, kPlocM = 3, kPlocTop = 4;
+
+__global__ void k_ploc_totals(const uint32_t *mflag, const uint32_t *moff, const uint32_t *kflag,
                               const uint32_t *koff, uint32_t *tot) {
-	tot[0] = moff[m - 1] + mflag[m - 1];
-	tot[1] = koff[m - 1] + kflag[m - 1];
+	const int m = (int)tot[kPlocM];
+	tot[kPlocMerges] = m > 1 ? moff[m - 1] + mflag[m - 1] : 0u;
+	tot[kPlocKept] = m > 1 ? koff[m - 1] + kflag[m - 1] : (uint32_t)m;
 }
 
-__global__ void k_ploc_merge(int m, int id_top, const int *nn, const uint32_t *mflag, const uint32_t *moff,
+__global__ void k_ploc_advance(uint32_t *tot) {
+	const int m = (int)tot[kPlocM];
+	if (m <= 1)
+		return;
+	if (tot[kPlocMerges] == 0u || tot[kPlocKept] != (uint32_t)m - tot[kPlocMerges])
+		tot[kPlocErr] = 1u;
+	tot[kPlocTop] = (uint32_t)((int)tot[kPlocTop] - (int)tot[kPlocMerges]);
+	tot[kPlocM] = tot[kPlocKept];
+}
+
+__global__ void k_ploc_merge(const uint32_t *tot, const int *nn, const uint32_t *mflag, const uint32_t *moff,
                              const uint32_t *kflag, const uint32_t *koff, const int *cref, const float4 *cbox,
                              const uint32_t *ccnt, int *cref2, float4 *cbox2, uint32_t *ccnt2, int *left, int *right,
                              uint32_t *count, float4 *box) {
 	int i = blockIdx.x * blockDim.x + threadIdx.x;
-	if (i >= m || !kflag[i])
+	const int m = (int)tot[kPlocM], id_top = (int)tot[kPlocTop];
+	if (m <= 1 || i >= m || !kflag[i])
 		return;
 	const uint32_t pos = koff[i];
 	float4 lo = cbox[2 * (size_t)i], hi = cbox[2 * (size_t)i + 1];
@@ -967,48 +995,54 @@ void ploc_tree(Bvh &b, const float *d_verts, int n, hipStream_t s) {
 	float4 *cbox = static_cast<float4 *>(b.ploc_box.reserve((size_t)n * 2 * 32));
 	uint32_t *ccnt = static_cast<uint32_t *>(b.ploc_cnt.reserve((size_t)n * 2 * 4));
 	int *nn = static_cast<int *>(b.ploc_nn.reserve((size_t)n * 4));
-	uint32_t *fl = static_cast<uint32_t *>(b.ploc_flags.reserve(((size_t)n * 4 + 4) * 4));
+	uint32_t *fl = static_cast<uint32_t *>(b.ploc_flags.reserve(((size_t)n * 4 + 8) * 4));
 	uint32_t *mflag = fl, *kflag = fl + n, *moff = fl + 2 * (size_t)n, *koff = fl + 3 * (size_t)n,
 	         *tot = fl + 4 * (size_t)n;
 	uint32_t *tmp = static_cast<uint32_t *>(b.scan_tmp.reserve(scan_tmp_words((size_t)n) * 4 + 16));
 	k_leaf_boxes<<<grid_for(n), kBlock, 0, s>>>(d_verts, order, n, box, cref, cbox, ccnt);
 	RN_HIP(hipGetLastError());
-	int m = n, cur = 0, id_top = n - 2;
-	uint32_t h[2];
-	while (m > 1) {
-		if (m <= kPlocTail) { // the rest in one work-group
-			int *st = reinterpret_cast<int *>(tot);
-			int hs[3] = {m, id_top, 0};
-			RN_HIP(hipMemcpyAsync(st, hs, 2 * sizeof(int), hipMemcpyHostToDevice, s));
+	// Rounds run in batches of kPlocBatch with m and id_top on the device
+	// (tot); the host reads them back once per batch. Kernels are sized for
+	// m_ub, the count at the last read-back, and stop at the device's m.
+	// Rounds after m <= kPlocTail within a batch are ordinary rounds: the
+	// tree does not depend on where the work-group tail takes over.
+	constexpr int kPlocBatch = 4;
+	int m_ub = n, cur = 0;
+	uint32_t hs[8] = {0u, 0u, 0u, (uint32_t)n, (uint32_t)(n - 2), 0u, 0u, 0u};
+	RN_HIP(hipMemcpyAsync(tot, hs, 5 * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+	while (m_ub > 1) {
+		if (m_ub <= kPlocTail) { // the rest in one work-group
+			int *st = reinterpret_cast<int *>(tot + kPlocM);
 			k_ploc_tail<<<1, kPlocTailBlock, 0, s>>>(n, cur, cref, cbox, ccnt, nn, mflag, kflag, left, right, count,
 			                                         box, st);
 			RN_HIP(hipGetLastError());
-			RN_HIP(hipMemcpyAsync(hs, st, 3 * sizeof(int), hipMemcpyDeviceToHost, s));
-			RN_HIP(hipStreamSynchronize(s));
-			if (hs[0] != 1)
-				throw Error("PLOC: a clustering round merged nothing (internal error)");
-			m = hs[0];
-			id_top = hs[1];
 			break;
 		}
-		int *cr = cref + (size_t)cur * n, *cr2 = cref + (size_t)(cur ^ 1) * n;
-		float4 *cb = cbox + (size_t)cur * 2 * n, *cb2 = cbox + (size_t)(cur ^ 1) * 2 * n;
-		uint32_t *cc = ccnt + (size_t)cur * n, *cc2 = ccnt + (size_t)(cur ^ 1) * n;
-		k_ploc_nn<<<grid_for(m), kBlock, 0, s>>>(m, cb, nn);
-		k_ploc_flags<<<grid_for(m), kBlock, 0, s>>>(m, nn, mflag, kflag);
-		exclusive_scan(mflag, moff, (size_t)m, tmp, s);
-		exclusive_scan(kflag, koff, (size_t)m, tmp, s);
-		k_ploc_totals<<<1, 1, 0, s>>>(m, mflag, moff, kflag, koff, tot);
-		k_ploc_merge<<<grid_for(m), kBlock, 0, s>>>(m, id_top, nn, mflag, moff, kflag, koff, cr, cb, cc, cr2, cb2,
-		                                            cc2, left, right, count, box);
-		RN_HIP(hipMemcpyAsync(h, tot, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+		for (int r = 0; r < kPlocBatch; r++) {
+			int *cr = cref + (size_t)cur * n, *cr2 = cref + (size_t)(cur ^ 1) * n;
+			float4 *cb = cbox + (size_t)cur * 2 * n, *cb2 = cbox + (size_t)(cur ^ 1) * 2 * n;
+			uint32_t *cc = ccnt + (size_t)cur * n, *cc2 = ccnt + (size_t)(cur ^ 1) * n;
+			k_ploc_nn<<<grid_for(m_ub), kBlock, 0, s>>>(tot + kPlocM, cb, nn);
+			k_ploc_flags<<<grid_for(m_ub), kBlock, 0, s>>>(tot + kPlocM, m_ub, nn, mflag, kflag);
+			exclusive_scan(mflag, moff, (size_t)m_ub, tmp, s);
+			exclusive_scan(kflag, koff, (size_t)m_ub, tmp, s);
+			k_ploc_totals<<<1, 1, 0, s>>>(mflag, moff, kflag, koff, tot);
+			k_ploc_merge<<<grid_for(m_ub), kBlock, 0, s>>>(tot, nn, mflag, moff, kflag, koff, cr, cb, cc, cr2, cb2,
+			                                            cc2, left, right, count, box);
+			k_ploc_advance<<<1, 1, 0, s>>>(tot);
+			cur ^= 1;
+		}
+		RN_HIP(hipMemcpyAsync(hs, tot, 5 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
 		RN_HIP(hipStreamSynchronize(s));
-		if (h[0] == 0 || (int)h[1] != m - (int)h[0])
+		if (hs[kPlocErr])
 			throw Error("PLOC: a clustering round merged nothing (internal error)");
-		id_top -= (int)h[0];
-		m = (int)h[1];
-		cur ^= 1;
+		m_ub = (int)hs[kPlocM];
 	}
+	RN_HIP(hipMemcpyAsync(hs, tot, 5 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+	RN_HIP(hipStreamSynchronize(s));
+	if (hs[kPlocErr] || hs[kPlocM] != 1u)
+		throw Error("PLOC: a clustering round merged nothing (internal error)");
+	const int id_top = (int)hs[kPlocTop];
 	if (id_top != -1)
 		throw Error("PLOC: node count mismatch (internal error)");
 }
