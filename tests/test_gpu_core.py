@@ -157,6 +157,30 @@ def test_bvh8_structure(rn, width, builder):
     assert canonical_tree(nodes, prims) == canonical_tree(nodes2, prims2)
 
 
+def test_flat_child_on_grid_line(rn):
+    """A node's grid origin puts a flat child on a grid line: the triangle in
+    the plane y = 0.3 keeps a zero-thickness box although its plane is not
+    the node's lower y (0.1), which with the origin at the lower corner would
+    lie 3.2 steps of 1/16 up and gain a step of thickness."""
+    v = np.array([[[0.0, 0.3, 0.0], [5.0, 0.3, 0.0], [0.0, 0.3, 5.0]],
+                  [[2.0, 0.1, 9.0], [10.0, 10.1, 3.0], [7.0, 4.0, 10.0]]], np.float32)
+    sc = rn.Scene.from_arrays(v, np.full((2, 3), 0.5, np.float32))
+    raw, prims = sc.export_bvh(8)
+    check_bvh(v, raw, prims, 8)
+    nd = np.frombuffer(raw.tobytes(), dtype=NODE)[0]
+    assert nd["imask"] == 0
+    sy = np.float32(2.0) ** (int(nd["e"][1]) - 127)
+    assert sy == np.float32(1 / 16)
+    flat = [s for s in range(8) if nd["meta"][s] and prims[int(nd["tri_base"]) + (nd["meta"][s] & 31)] == 0]
+    assert len(flat) == 1
+    s = flat[0]
+    assert nd["qlo"][1, s] == nd["qhi"][1, s]
+    assert nd["p"][1] + np.float32(nd["qlo"][1, s]) * sy == np.float32(0.3)
+    # the ray along the plane's normal from just below does not reach the box
+    prim, t, _, _ = sc.trace_rays(np.array([[1.0, 0.2, 1.0]], np.float32), np.array([[0, 1, 0]], np.float32))
+    assert prim[0] == 0 and abs(t[0] - 0.1) < 1e-6
+
+
 @pytest.mark.parametrize("builder", ["ploc", "lbvh"])
 def test_closest_and_any_vs_brute(rn, oracle, builder):
     v = soup(3000, seed=5)
