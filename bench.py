@@ -80,6 +80,35 @@ def rel_rmse(img, fixture):
     return float(np.sqrt(((got - ref) ** 2).sum() / (ref ** 2).sum()))
 
 
+def oracle_fixture(name):
+    """The FP64 65,536-spp CPU-oracle pixel subset of a config
+    (tests/golden/make_oracle_rmse.py), or None."""
+    fx = os.path.join(ROOT, "tests", "golden", f"oracle64k_{name}.npz")
+    return np.load(fx) if os.path.isfile(fx) else None
+
+
+def subset_note(f):
+    return f"{len(f['pixels'])} pixels, one seeded pixel per {int(f['block'])}x{int(f['block'])} block"
+
+
+def config0_rmse(rn):
+    """Config 0 (the 32-triangle Cornell box, 320x240, 16 spp, depth 5, RR
+    from depth 3, Philox seed 0) from both integrators against its oracle."""
+    from rednoise_amd import scenes
+    f = oracle_fixture("config0")
+    if f is None:
+        return None
+    sd = scenes.cornell_box()
+    sc = rn.Scene.from_arrays(sd.verts, sd.albedo, sd.emission)
+    cam = rn.Camera.make(**sd.camera)
+    out = {"subset": subset_note(f)}
+    for key, integ in (("wavefront_rel_rmse", rn.INTEGRATOR_WAVEFRONT), ("plain_rel_rmse", rn.INTEGRATOR_MEGAKERNEL)):
+        p = rn.RenderParams.make(320, 240, 16, 5, 3, seed=0, ray_eps=0.05, integrator=integ)
+        out[key] = round(rel_rmse(sc.render_hdr(cam, p), f), 5)
+    sc.close()
+    return out
+
+
 def adjoint_bench(rn, lib, torch, n_fd=64):
     from rednoise_amd import scenes
     tgt, start = scenes.config3_scenes()
@@ -136,6 +165,8 @@ def adjoint_bench(rn, lib, torch, n_fd=64):
         g_ad[k] = ga[t, c]
     sc.set_materials(albedo=alb)
     err = float(np.linalg.norm(g_ad - g_fd) / max(1e-30, np.linalg.norm(g_fd)))
+    f = oracle_fixture("config3")
+    rmse = {"subset": subset_note(f), "forward_rel_rmse": round(rel_rmse(img, f), 5)} if f is not None else None
     return {
         "workload": f"config3: {n} tris (cube sphere m=91 + 12 Cornell), 512x512, 32 spp, depth 6, no roulette",
         "ms_per_iter": round(fwd_ms + adj_ms, 3), "forward_ms": round(fwd_ms, 3), "adjoint_ms": round(adj_ms, 3),
@@ -143,6 +174,7 @@ def adjoint_bench(rn, lib, torch, n_fd=64):
         "kernel_adjoint_over_forward": round(kern_adj / kern_fwd, 3),
         "grad_rel_err_vs_fd": err, "fd_h": h, "fd_triangles": n_fd,
         "target": "t_adjoint <= 3 t_forward and rel err <= 1e-2",
+        "equal_rmse": rmse,
     }
 
 
@@ -173,13 +205,33 @@ def batch_bench(rn, torch):
         sc.build(8)
     for sc, cam, p in items[:2]:
         sc.render_hdr(cam, p)
+    f = oracle_fixture("config2")
+    keep = set(int(k) for k in f["scenes"]) if f is not None else set()
+    plain_imgs = {}
     t0 = time.perf_counter()
     plain_rays = 0
-    for sc, cam, p in items:
-        sc.render_hdr(cam, p)
+    for k, (sc, cam, p) in enumerate(items):
+        img = sc.render_hdr(cam, p)
+        if k in keep:
+            plain_imgs[k] = img
         s2 = rn.last_stats()
         plain_rays += s2["rays_closest"] + s2["rays_shadow"]
     dt_plain = time.perf_counter() - t0
+    rmse = None
+    if f is not None:
+        def subset_rmse(get):
+            num = den = 0.0
+            for k in keep:
+                sel = f["scene"] == k
+                ref = f["values"][sel]
+                got = get(k).reshape(-1, 3)[f["pixels"][sel]].astype(np.float64)
+                num += float(((got - ref) ** 2).sum())
+                den += float((ref ** 2).sum())
+            return float(np.sqrt(num / den))
+        rmse = {"subset": f"scenes {sorted(keep)}, {len(f['pixels'])} pixels, one seeded pixel per "
+                          f"{int(f['block'])}x{int(f['block'])} block of each",
+                "batched_rel_rmse": round(subset_rmse(lambda k: imgs[k]), 5),
+                "plain_rel_rmse": round(subset_rmse(lambda k: plain_imgs[k]), 5)}
     return {"workload": f"config2: 64 scenes, {sum(sc.triangle_count for sc, _, _ in items):,} triangles, "
                         f"{sum(p.width * p.height * p.spp for _, _, p in items) / 1e6:.1f}M paths, depth 8, "
                         "one persistent wavefront, hipGraph replay",
@@ -188,7 +240,7 @@ def batch_bench(rn, torch):
             "batched_over_plain": round((rays / dt) / (plain_rays / dt_plain), 3),
             "autotune": {"best": report["best"], "candidates": len(report["candidates"]),
                          "trial_spp_fraction": report["trial_spp_fraction"]},
-            "scene_generation_s": round(gen_s, 1)}
+            "scene_generation_s": round(gen_s, 1), "equal_rmse": rmse}
 
 
 def terrain_bench(rn, lib, torch):
@@ -214,6 +266,10 @@ def terrain_bench(rn, lib, torch):
            "build_ms": round(st[0]["build_ms"], 2), "bvh": {"nodes": st[0]["bvh_nodes"], "depth": st[0]["bvh_depth"]},
            "rays": {"closest": st[0]["rays_closest"], "shadow": st[0]["rays_shadow"], "paths": st[0]["paths"]},
            "scene_generation_s": round(gen_s, 1)}
+    f = oracle_fixture("config4")
+    if f is not None:
+        res["equal_rmse"] = {"subset": subset_note(f),
+                             "wavefront_rel_rmse": round(rel_rmse(out.cpu().numpy(), f), 5)}
     del out
     specs = scenes.config2_batch(16, seed=2)
     pt = rn.RenderParams.make(W, H, 64, depth, rr, seed=0, ray_eps=eps)
@@ -552,14 +608,20 @@ def main():
         "rays_per_step": {"closest": st_w[0]["rays_closest"], "shadow": st_w[0]["rays_shadow"],
                           "paths": st_w[0]["paths"]},
     }
-    fx = os.path.join(ROOT, "tests", "golden", "oracle64k_config1.npz")
-    if os.path.isfile(fx):
-        f = np.load(fx)
-        result["equal_rmse"] = {"oracle": f"FP64 CPU oracle, {int(f['spp'])} spp, {len(f['pixels'])} pixels "
-                                          f"(one seeded pixel per {int(f['block'])}x{int(f['block'])} block)",
+    f = oracle_fixture("config1")
+    if f is not None:
+        result["equal_rmse"] = {"oracle": f"FP64 CPU oracle, {int(f['spp'])} spp, independent Philox seed; "
+                                          "configs 2-4 carry theirs in their own entries",
+                                "subset": subset_note(f),
                                 "wavefront_rel_rmse": round(rel_rmse(img_w, f), 5),
                                 "plain_rel_rmse": round(rel_rmse(img_m, f), 5)}
     if not args.skip_extras:
+        try:
+            c0 = config0_rmse(rn)
+            if c0 is not None and "equal_rmse" in result:
+                result["equal_rmse"]["config0"] = c0
+        except Exception as e:
+            result.setdefault("equal_rmse", {})["config0"] = {"error": str(e)}
         try:
             result["adjoint"] = adjoint_bench(rn, lib, torch)
             log(f"adjoint: {result['adjoint']}")

@@ -40,26 +40,60 @@ def pixel_subset(W, H, block, seed):
 WORKLOADS = {
     # name: (scene builder, W, H, max_depth, rr_depth, ray_eps, block)
     "config0": (lambda: scenes.cornell_box(), 320, 240, 5, 3, 0.05, 8),
-    "config1": (lambda: scenes.config1_scene(), 1920, 1080, 8, 3, 0.03, 32),
+    "config1": (lambda: scenes.config1_scene(), 1920, 1080, 8, 3, 0.03, 16),
+    # the adjoint's forward render: the start scene (seed-2 albedos, emission
+    # x 0.5), no roulette
+    "config3": (lambda: scenes.config3_scenes()[1], 512, 512, 6, 1 << 30, 0.03, 16),
+    # 10M triangles: one pixel per 32 x 32 block (8,160 pixels) keeps the CPU
+    # reference to minutes
+    "config4": (lambda: scenes.terrain_scene(), 3840, 2160, 12, 3, 0.05, 32),
 }
+# config 2: every 8th of the 64 batch scenes (scenes.config2_batch(64, seed=2)),
+# one pixel per 64 x 64 block of each
+CONFIG2_SCENES = list(range(0, 64, 8))
+CONFIG2_BLOCK = 64
+
+
+def render_subset(sd, W, H, D, rr, eps, block, seed=17):
+    cam = O.camera_basis(sd.camera["eye"], sd.camera["target"], sd.camera["up"], sd.camera["vfov_deg"], W, H)
+    pix = pixel_subset(W, H, block, seed=seed)
+    ps = O.PhysScene(sd.verts, sd.albedo, sd.emission, "f64")
+    p = O.params(W, H, SPP, D, rr, 0.0, ORACLE_SEED, 0, eps)
+    out, cnt = ps.render(cam, p, pixels=pix)
+    return pix, out, cnt
 
 
 def main(names):
     for name in names:
+        t0 = time.time()
+        if name == "config2":
+            specs = scenes.config2_batch(64, seed=2)
+            idx, pixs, vals, widths = [], [], [], []
+            for k in CONFIG2_SCENES:
+                s = specs[k]
+                pix, out, _ = render_subset(s.scene, s.width, s.height, s.max_depth, 3, 0.03, CONFIG2_BLOCK)
+                idx.append(np.full(len(pix), k, np.int32))
+                pixs.append(pix)
+                vals.append(out)
+                widths.append(s.width)
+                print(f"  config2 scene {k}: {s.width}x{s.height}, {len(pix)} pixels, {time.time() - t0:.0f}s",
+                      flush=True)
+            dt = time.time() - t0
+            np.savez_compressed(os.path.join(HERE, "oracle64k_config2.npz"), scene=np.concatenate(idx),
+                                pixels=np.concatenate(pixs), values=np.concatenate(vals), spp=SPP, seed=ORACLE_SEED,
+                                scenes=np.array(CONFIG2_SCENES, np.int32), block=CONFIG2_BLOCK, max_depth=8,
+                                rr_depth=3, ray_eps=0.03, seconds=dt, threads=os.cpu_count())
+            print(f"config2: {sum(len(x) for x in pixs)} pixels x {SPP} spp in {dt:.0f}s", flush=True)
+            continue
         build, W, H, D, rr, eps, block = WORKLOADS[name]
         sd = build()
-        cam = O.camera_basis(sd.camera["eye"], sd.camera["target"], sd.camera["up"], sd.camera["vfov_deg"], W, H)
-        pix = pixel_subset(W, H, block, seed=17)
-        ps = O.PhysScene(sd.verts, sd.albedo, sd.emission, "f64")
-        p = O.params(W, H, SPP, D, rr, 0.0, ORACLE_SEED, 0, eps)
-        t0 = time.time()
-        out, cnt = ps.render(cam, p, pixels=pix)
+        pix, out, cnt = render_subset(sd, W, H, D, rr, eps, block)
         dt = time.time() - t0
         np.savez_compressed(os.path.join(HERE, f"oracle64k_{name}.npz"), pixels=pix, values=out, spp=SPP,
                             seed=ORACLE_SEED, width=W, height=H, max_depth=D, rr_depth=rr, ray_eps=eps, block=block,
                             seconds=dt, threads=os.cpu_count())
-        print(f"{name}: {len(pix)} pixels x {SPP} spp in {dt:.0f}s ({cnt.closest + cnt.shadow} rays)")
+        print(f"{name}: {len(pix)} pixels x {SPP} spp in {dt:.0f}s ({cnt.closest + cnt.shadow} rays)", flush=True)
 
 
 if __name__ == "__main__":
-    main(sys.argv[1:] or list(WORKLOADS))
+    main(sys.argv[1:] or list(WORKLOADS) + ["config2"])

@@ -356,3 +356,51 @@ def test_adjoint_matches_finite_differences(oracle):
             fd = (loss(sd.albedo, ep) - loss(sd.albedo, em_)) / step
             assert abs(fd - ge[tri, ch]) <= 1e-4 * max(1.0, abs(fd)), (tri, ch, fd, ge[tri, ch])
     assert checked == 18
+
+
+def test_rmse_fixtures_match_their_workloads(oracle):
+    """The FP64 65,536-spp pixel subsets that bench.py scores every config
+    against carry the parameters of tests/golden/make_oracle_rmse.py, hold
+    finite non-negative radiance, and config 0's agree with a fresh oracle
+    render of 6 of their pixels at 4,096 spp on another seed (within 5
+    standard errors of that render)."""
+    import sys
+    sys.path.insert(0, GOLDEN)
+    import make_oracle_rmse as M
+    from rednoise_amd import scenes
+    for name, (_, W, H, D, rr, eps, block) in M.WORKLOADS.items():
+        path = os.path.join(GOLDEN, f"oracle64k_{name}.npz")
+        if not os.path.isfile(path):
+            pytest.skip(f"{path} not generated")
+        f = np.load(path)
+        assert int(f["spp"]) == M.SPP and int(f["block"]) == block
+        assert (int(f["width"]), int(f["height"]), int(f["max_depth"]), int(f["rr_depth"])) == (W, H, D, rr)
+        assert abs(float(f["ray_eps"]) - eps) < 1e-9
+        assert np.array_equal(f["pixels"], M.pixel_subset(W, H, block, seed=17))
+        assert np.isfinite(f["values"]).all() and (f["values"] >= 0).all()
+    path2 = os.path.join(GOLDEN, "oracle64k_config2.npz")
+    if not os.path.isfile(path2):
+        pytest.skip(f"{path2} not generated")
+    f2 = np.load(path2)
+    specs = scenes.config2_batch(64, seed=2)
+    for k in M.CONFIG2_SCENES:
+        s = specs[k]
+        sel = f2["scene"] == k
+        assert np.array_equal(f2["pixels"][sel], M.pixel_subset(s.width, s.height, M.CONFIG2_BLOCK, seed=17))
+    # config 0 against a fresh render of a few of its pixels
+    f0 = np.load(os.path.join(GOLDEN, "oracle64k_config0.npz"))
+    sd = scenes.cornell_box()
+    cam = oracle.camera_basis(sd.camera["eye"], sd.camera["target"], sd.camera["up"], sd.camera["vfov_deg"], 320, 240)
+    ps = oracle.PhysScene(sd.verts, sd.albedo, sd.emission, "f64")
+    pick = np.arange(0, len(f0["pixels"]), len(f0["pixels"]) // 6)[:6]
+    spp = 4096
+    runs = []
+    for seed in (101, 202, 303, 404):
+        p = oracle.params(320, 240, spp // 4, 5, 3, 0.0, seed, 0, 0.05)
+        out, _ = ps.render(cam, p, pixels=f0["pixels"][pick])
+        runs.append(out)
+    runs = np.array(runs)
+    mean = runs.mean(0)
+    se = runs.std(0, ddof=1) / 2.0 + 1e-3 * np.abs(mean) + 1e-6
+    z = np.abs(mean - f0["values"][pick]) / se
+    assert (z < 5).all(), z
