@@ -448,6 +448,33 @@ def reference_modeR(rn, torch):
     return res
 
 
+def primary_parity(rn):
+    """Primary rays of rn_render's camera through the GPU BVH8 with the
+    reference's triangle test, against the reference's own BVH::intersect
+    (tests/golden/primary_*.npz): ids compared for every pixel; mismatches
+    are split by the 1e-6 barycentric edge band (a hit within 1e-6 of an
+    edge on either side may legitimately go to the neighbouring triangle)."""
+    out = {}
+    for key, asset, scale in (("cornell", "cornell-box.obj", 0.35), ("quad", "textured-quad.obj", 0.35),
+                              ("sphere", "sphere.obj", 1.0)):
+        g = np.load(os.path.join(ROOT, "tests", "golden", f"primary_{key}.npz"))
+        sc = rn.Scene.load_obj(os.path.join(ROOT, "tests", "golden", "assets", asset), scale)
+        prim, t, u, v, _ = sc.trace_primary_compat(int(g["width"]), int(g["height"]), float(g["cam_z"]))
+        same = prim == g["prim"]
+
+        def band(uu, vv):
+            return np.minimum(np.minimum(uu, vv), 1.0 - uu - vv) < 1e-6
+        inb = band(g["u"], g["v"]) | band(u, v)
+        hit = same & (prim >= 0)
+        out[key] = {"rays": int(prim.size), "hits": int((g["prim"] >= 0).sum()), "id_mismatches": int((~same).sum()),
+                    "in_edge_band": int((~same & inb).sum()), "outside_edge_band": int((~same & ~inb).sum()),
+                    "tuv_bit_identical_of_matching_hits": round(float(
+                        ((t[hit] == g["t"][hit]) & (u[hit] == g["u"][hit]) & (v[hit] == g["v"][hit])).mean())
+                        if hit.any() else 1.0, 6)}
+        sc.close()
+    return out
+
+
 def estimators_bench(rn):
     """Photon mapping and bidirectional path tracing on the Cornell asset at
     the reference CLI's defaults (render -m photon: 640x480, 200000 photons,
@@ -653,6 +680,11 @@ def main():
                 result["reference_rn_render"] = r
         except Exception as e:
             result["reference_rn_render"] = {"error": str(e)}
+        try:
+            result["primary_ray_parity"] = primary_parity(rn)
+            log(f"primary_ray_parity: {result['primary_ray_parity']}")
+        except Exception as e:
+            result["primary_ray_parity"] = {"error": str(e)}
         try:
             result["estimators"] = estimators_bench(rn)
             log(f"estimators: {result['estimators']}")
