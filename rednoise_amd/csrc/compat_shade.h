@@ -7,6 +7,7 @@
 
 #include "rn_compat.h"
 #include "rn_internal.h"
+#include "rn_kernels.h"
 
 namespace rn {
 
@@ -154,6 +155,81 @@ RN_DEV float3 reflectv(float3 i, float3 n) { return greflect(i, n); }
 #define RN_LPOS f3(0.0f, 0.95f, 0.0f)
 constexpr float kLRadius = 0.15f, kLIntensity = 40.0f, kLKq = 12.566370f;
 
+// The 9 any-hit queries of visibility as one packet: a node is loaded once
+// for every ray whose box test reached it, and a leaf triangle once for the
+// rays that reached its leaf. Each ray still meets exactly the boxes and
+// triangles of its own traversal (a child is entered with the rays that hit
+// its box, and a ray leaves the packet at its first hit), so each answer is
+// the one traverse<1, 1> gives and the count is exact. Returns the rays that
+// reach their light point unoccluded. Entries of the depth-first stack hold
+// a node and its ray mask; a level pops one entry and pushes at most 8.
+#ifndef RN_VIS_PACKET
+#define RN_VIS_PACKET 1
+#endif
+constexpr int kVisRays = 9;
+constexpr int kPacketStack = 7 * kMaxBvhDepth + 1;
+
+RN_DEV int packet_unoccluded(const Bvh8Node *__restrict__ nodes, const float4 *__restrict__ tris, float3 o,
+                             const float3 (&dir)[kVisRays], const float (&tmax)[kVisRays], int ignore) {
+	TRay r[kVisRays];
+#pragma unroll
+	for (int i = 0; i < kVisRays; i++)
+		r[i] = make_tray(o, dir[i]);
+	uint32_t active = (1u << kVisRays) - 1u;
+	uint64_t st[kPacketStack];
+	int sp = 0;
+	st[sp++] = (uint64_t)active << 32; // the root, node 0
+	while (sp > 0 && active) {
+		const uint64_t e = st[--sp];
+		const uint32_t node = (uint32_t)e;
+		const uint32_t m = (uint32_t)(e >> 32) & active;
+		if (!m)
+			continue;
+		uint32_t w[20];
+		load_node_g(nodes, node, w);
+		uint32_t th[kVisRays], tri_any = 0, smask[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+		for (int i = 0; i < kVisRays; i++) {
+			th[i] = 0;
+			if ((m >> i) & 1u) {
+				uint32_t ih, t;
+				intersect_children(w, r[i], tmax[i], ih, t, nullptr);
+				const uint32_t slots = permute_oct(ih, r[i].oct); // back to slot order
+				th[i] = t;
+				tri_any |= t;
+#pragma unroll
+				for (int s = 0; s < 8; s++)
+					smask[s] |= ((slots >> s) & 1u) << i;
+			}
+		}
+		const uint32_t tri_base = w[5];
+		while (tri_any && active) {
+			const uint32_t k = __builtin_ctz(tri_any);
+			tri_any &= tri_any - 1u;
+			const uint32_t ti = tri_base + k;
+			const float4 a = gload_f4(tris, 3 * ti), b4 = gload_f4(tris, 3 * ti + 1), c4 = gload_f4(tris, 3 * ti + 2);
+			if (__float_as_int(a.w) == ignore)
+				continue;
+#pragma unroll
+			for (int i = 0; i < kVisRays; i++) {
+				if (((active & m) >> i) & 1u && ((th[i] >> k) & 1u)) {
+					float tt, uu, vv;
+					if (tri_compat(o, dir[i], xyz(a), xyz(b4), xyz(c4), tt, uu, vv) && tt < tmax[i])
+						active &= ~(1u << i);
+				}
+			}
+		}
+		const uint32_t imask = w[3] >> 24, child_base = w[4];
+#pragma unroll
+		for (int s = 0; s < 8; s++) {
+			const uint32_t cm = smask[s] & active;
+			if (cm && sp < kPacketStack)
+				st[sp++] = ((uint64_t)cm << 32) | (uint64_t)(child_base + __builtin_popcount(imask & ((1u << s) - 1u)));
+		}
+	}
+	return __builtin_popcount(active);
+}
+
 // src/Renderer.cpp:483-497 (the default light's 9-point disk) in glm's
 // operation order: normalize, cross, length and the division to / dist.
 template <int LDS, int SPILL>
@@ -167,15 +243,28 @@ RN_DEV float visibility(const DevScene &sc, float3 point, int ignore, uint64_t *
 	float3 tangent = gnormalize(gcross(up, axis));
 	float3 bitangent = gcross(axis, tangent);
 	int un = 0;
+	if (RN_VIS_PACKET) {
+		float3 dir[kVisRays];
+		float dist[kVisRays];
+#pragma unroll
+		for (int i = 0; i < kVisRays; i++) {
+			float3 spos = gadd(lp, gscale(gadd(gscale(tangent, ks[i][0]), gscale(bitangent, ks[i][1])), kLRadius));
+			float3 to = gsub(spos, point);
+			dist[i] = sqrtf(gdot(to, to));
+			dir[i] = f3(to.x / dist[i], to.y / dist[i], to.z / dist[i]);
+		}
+		un = packet_unoccluded(sc.nodes, sc.tris, point, dir, dist, ignore);
+	} else {
 #pragma unroll 1
-	for (int i = 0; i < 9; i++) {
-		float3 spos = gadd(lp, gscale(gadd(gscale(tangent, ks[i][0]), gscale(bitangent, ks[i][1])), kLRadius));
-		float3 to = gsub(spos, point);
-		float dist = sqrtf(gdot(to, to));
-		float3 dir = f3(to.x / dist, to.y / dist, to.z / dist);
-		Hit h = traverse<1, 1, LDS, SPILL>(sc.nodes, sc.tris, point, dir, dist, ignore, lds, kCBlock, lane);
-		if (h.prim == 0)
-			un++;
+		for (int i = 0; i < 9; i++) {
+			float3 spos = gadd(lp, gscale(gadd(gscale(tangent, ks[i][0]), gscale(bitangent, ks[i][1])), kLRadius));
+			float3 to = gsub(spos, point);
+			float dist = sqrtf(gdot(to, to));
+			float3 dir = f3(to.x / dist, to.y / dist, to.z / dist);
+			Hit h = traverse<1, 1, LDS, SPILL>(sc.nodes, sc.tris, point, dir, dist, ignore, lds, kCBlock, lane);
+			if (h.prim == 0)
+				un++;
+		}
 	}
 	return (float)un / 9.0f;
 }
