@@ -177,7 +177,7 @@ __global__ __launch_bounds__(256) void k_generate(WQ q, int cur) {
 #define RN_CHUNKS_PER_WAVE 16
 #endif
 #ifndef RN_CHUNK_MIN
-#define RN_CHUNK_MIN 128
+#define RN_CHUNK_MIN 256 // 128 / 256 / 384 / 512: 51.0-52.0 / 50.4-51.6 / 51.7 / 52.0 ms per 1M-triangle 1080p frame
 #endif
 #ifndef RN_CHUNK_MAX
 #define RN_CHUNK_MAX 1024
