@@ -214,31 +214,28 @@ def test_closest_and_any_vs_brute(rn, oracle, builder):
     assert (occ2[prim >= 0] == 1).all()
 
 
-def test_flat_scene_closest_and_any_vs_brute(rn, oracle):
-    """The Cornell box (axis-aligned walls and light on grid lines of their
-    nodes, so their boxes have zero thickness): rays leaving the walls the
-    way the path tracer's secondary rays do (origin 0.05 off the surface)
-    and rays from the interior give the brute-force closest hit bit for bit,
-    and shadow rays toward the light are occluded exactly when the closest
-    hit comes first."""
+def _surface_rays_vs_brute(rn, oracle, sd, n_surf, n_in, seed):
+    """Rays leaving the scene's surfaces the way the path tracer's secondary
+    rays do (origin 0.05 off the surface, on the side the ray goes) and rays
+    from the box interior give the brute-force closest hit bit for bit, and
+    shadow rays toward the light are occluded exactly when the closest hit
+    comes first."""
     import ctypes as C
-    from rednoise_amd import scenes
-    sd = scenes.cornell_box()
     v = sd.verts
     sc = rn.Scene.from_arrays(v, sd.albedo, sd.emission)
-    rng = np.random.default_rng(21)
-    tri = rng.integers(0, len(v), 3000)
-    b = rng.dirichlet((1, 1, 1), 3000).astype(np.float32)
+    rng = np.random.default_rng(seed)
+    tri = rng.integers(0, len(v), n_surf)
+    b = rng.dirichlet((1, 1, 1), n_surf).astype(np.float32)
     p = (b[:, :1] * v[tri, 0] + b[:, 1:2] * v[tri, 1] + b[:, 2:] * v[tri, 2]).astype(np.float32)
-    n = np.cross(v[tri, 1] - v[tri, 0], v[tri, 2] - v[tri, 0])
+    n = np.cross(v[tri, 1] - v[tri, 0], v[tri, 2] - v[tri, 0]).astype(np.float64)
     n /= np.linalg.norm(n, axis=1, keepdims=True)
-    d = rng.normal(size=(3000, 3))
+    d = rng.normal(size=(n_surf, 3))
     d /= np.linalg.norm(d, axis=1, keepdims=True)
-    n = np.where((n * d).sum(1, keepdims=True) < 0, -n, n)  # leave on the side the ray goes
+    n = np.where((n * d).sum(1, keepdims=True) < 0, -n, n)
     o_surf = (p + 0.05 * n).astype(np.float32)
-    o_in = rng.uniform([30, 30, 30], [525, 525, 525], size=(1000, 3)).astype(np.float32)
+    o_in = rng.uniform([30, 30, 30], [525, 525, 525], size=(n_in, 3)).astype(np.float32)
     o = np.concatenate([o_surf, o_in])
-    d = np.concatenate([d, rng.normal(size=(1000, 3))])
+    d = np.concatenate([d, rng.normal(size=(n_in, 3))])
     d = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
     prim, t, u, w = sc.trace_rays(o, d)
     ps = oracle.PhysScene(v, sd.albedo, sd.emission, "f32")
@@ -253,10 +250,9 @@ def test_flat_scene_closest_and_any_vs_brute(rn, oracle):
         if bp >= 0:
             assert t[i] == tt.value and u[i] == uu.value and w[i] == vv.value, i
     assert mism == 0, f"{mism} closest-hit mismatches"
-    # shadow rays from the surface points to random points of the light
     light = np.flatnonzero(sd.emission.max(1) > 0)
-    lt = rng.choice(light, len(o_surf))
-    lb = rng.dirichlet((1, 1, 1), len(o_surf)).astype(np.float32)
+    lt = rng.choice(light, n_surf)
+    lb = rng.dirichlet((1, 1, 1), n_surf).astype(np.float32)
     q = (lb[:, :1] * v[lt, 0] + lb[:, 1:2] * v[lt, 1] + lb[:, 2:] * v[lt, 2]).astype(np.float32)
     to = q - o_surf
     dist = np.linalg.norm(to, axis=1)
@@ -265,6 +261,21 @@ def test_flat_scene_closest_and_any_vs_brute(rn, oracle):
     occ = sc.trace_rays(o_surf, sdir, tmax=tmax, query=rn.QUERY_ANY)
     cp, ct, _, _ = sc.trace_rays(o_surf, sdir)
     assert np.array_equal(occ.astype(bool), (cp >= 0) & (ct < tmax))
+
+
+def test_flat_scene_closest_and_any_vs_brute(rn, oracle):
+    """The Cornell box: axis-aligned walls and the light lie on grid lines of
+    their nodes, so their boxes have zero thickness."""
+    from rednoise_amd import scenes
+    _surface_rays_vs_brute(rn, oracle, scenes.cornell_box(), 3000, 1000, 21)
+
+
+def test_bench_scene_closest_and_any_vs_brute(rn, oracle):
+    """The 1,048,588-triangle scene of the throughput benchmark (displaced
+    sphere in the Cornell walls): 384 surface rays and 128 interior rays
+    against brute force over every triangle."""
+    from rednoise_amd import scenes
+    _surface_rays_vs_brute(rn, oracle, scenes.config1_scene(0), 384, 128, 22)
 
 
 def _edge_band(u, v, eps=1e-6):
