@@ -14,8 +14,10 @@ Also reported: the adjoint (config 3: 99,372-triangle sphere + Cornell walls,
 512x512, 32 spp, depth 6) ms/iteration, its ratio to the forward pass and its
 gradient error against central finite differences (h = 1e-3, 64 random
 triangles); the CPU oracle (FP32 port of the estimator) timed on a bounded
-pixel sample of the same workload on this host's cores; and, when present,
-the compiled reference's rn_render(RN_PATHTRACED) beside ours.
+pixel sample of the same workload on this host's cores; and rn_render(
+RN_PATHTRACED) beside the reference's CPU time for the same call (this host's
+port time scaled by the port/reference ratio measured in the build
+container).
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -395,55 +397,64 @@ def cpu_baseline(sd, W, H, spp, depth, rr, eps, budget_s=12.0):
             "samples_per_s": round(total_paths / total_t, 1)}
 
 
+def reference_calibration():
+    """Port/reference timings measured in the build container
+    (tests/golden/calibrate_reference.py), or None."""
+    fx = os.path.join(ROOT, "tests", "golden", "reference_timing.json")
+    return json.load(open(fx)) if os.path.isfile(fx) else None
+
+
+def _md5(a) -> str:
+    import hashlib
+    return hashlib.md5(np.ascontiguousarray(a, np.uint8).tobytes()).hexdigest()
+
+
 def reference_modeR(rn, torch):
-    """The compiled reference rn_render(RN_PATHTRACED) (oracle/_ref) on the
-    Cornell asset beside this library's rn_render, same arguments."""
+    """rn_render(RN_PATHTRACED) and rn_render(RN_RAYTRACED) on the Cornell
+    asset beside the reference's CPU time for the same call. The compiled
+    reference does not run here: its time on this host is the CPU port's time
+    here (the oracle's restatement with the reference's own random streams,
+    OpenMP over this host's cores) divided by the port/reference ratio
+    measured on the same call in the build container; the GPU image is
+    compared with the reference image's MD5 recorded there."""
     from oracle import oracle as O
-    ref = O.ref()
-    if ref is None:
+    cal = reference_calibration()
+    if cal is None:
         return None
     asset = os.path.join(ROOT, "tests", "golden", "assets", "cornell-box.obj")
-    W, H, spp = 320, 240, 16
-    sc = ref.rn_scene_load_obj(asset.encode(), C.c_float(0.35))
-    buf = np.empty(W * H * 4, np.uint8)
-    t0 = time.perf_counter()
-    ref.rn_render(sc, 3, W, H, C.c_float(4.0), spp, buf)
-    t_ref = time.perf_counter() - t0
-    ref.rn_scene_free(sc)
+    c = cal["rn_render_pathtraced"]
+    W, H, spp, cz = c["width"], c["height"], c["spp"], c["cam_z"]
     ours = rn.Scene.load_obj(asset, 0.35)
-    ours.render("pathtraced", W, H, 4.0, spp)
+    ours.render("pathtraced", W, H, cz, spp)
     t0 = time.perf_counter()
-    img = ours.render("pathtraced", W, H, 4.0, spp).to_numpy()
+    img = ours.render("pathtraced", W, H, cz, spp).to_numpy()
     t_gpu = time.perf_counter() - t0
-    same = float((img.reshape(-1, 4) == buf.reshape(-1, 4)).all(axis=1).mean())
-    # calibration of the CPU port against the reference on the same call
     cs = O.CompatArrays.from_scene(ours)
     port = np.zeros(W * H * 4, np.uint8)
+    O.lib().or_compat_render_mt(C.byref(cs.s), W, H, cz, spp, 0, H, port)  # warm-up
     t0 = time.perf_counter()
-    O.lib().or_compat_render_mt(C.byref(cs.s), W, H, 4.0, spp, 0, H, port)
+    O.lib().or_compat_render_mt(C.byref(cs.s), W, H, cz, spp, 0, H, port)
     t_port = time.perf_counter() - t0
-    res = {"call": f"rn_render(RN_PATHTRACED) cornell-box.obj 320x240 spp={spp} cam_z=4",
-           "reference_cpu_s": round(t_ref, 4), "reference_threads": os.cpu_count(), "gpu_s": round(t_gpu, 5),
-           "speedup": round(t_ref / t_gpu, 1), "identical_pixels": round(same, 5),
-           "cpu_port_s": round(t_port, 4), "cpu_port_over_reference_time": round(t_port / t_ref, 3),
-           "cpu_port_identical_pixels": round(float((port.reshape(-1, 4) == buf.reshape(-1, 4)).all(axis=1).mean()),
-                                              5)}
-    # the Whitted mode on the same asset, 1280x960
-    W2, H2 = 1280, 960
-    sc = ref.rn_scene_load_obj(asset.encode(), C.c_float(0.35))
-    buf2 = np.empty(W2 * H2 * 4, np.uint8)
+    t_ref = t_port / c["port_over_reference"]
+    res = {"call": c["call"], "gpu_s": round(t_gpu, 5),
+           "identical_to_reference_image": _md5(img) == c["reference_rgba_md5"],
+           "cpu_port_s": round(t_port, 4), "cpu_port_threads": os.cpu_count(),
+           "cpu_port_identical_pixels": round(float((port.reshape(-1, 4) == img.reshape(-1, 4)).all(axis=1).mean()), 5),
+           "reference_cpu_s": round(t_ref, 4), "speedup": round(t_ref / t_gpu, 1),
+           "reference_timing": (f"container-calibrated: the port's time on this host / {c['port_over_reference']} "
+                                f"(port/reference time on the same call in the build container, "
+                                f"{cal['container_threads']} threads: {c['port_s']} s / {c['reference_s']} s)")}
+    c2 = cal["rn_render_raytraced"]
+    W2, H2 = c2["width"], c2["height"]
+    ours.render("raytraced", W2, H2, cz, 1)
     t0 = time.perf_counter()
-    ref.rn_render(sc, 2, W2, H2, C.c_float(4.0), 1, buf2)
-    t_ref2 = time.perf_counter() - t0
-    ref.rn_scene_free(sc)
-    ours.render("raytraced", W2, H2, 4.0, 1)
-    t0 = time.perf_counter()
-    img = ours.render("raytraced", W2, H2, 4.0, 1).to_numpy()
+    img = ours.render("raytraced", W2, H2, cz, 1).to_numpy()
     t_gpu2 = time.perf_counter() - t0
-    same = float((img.reshape(-1, 4) == buf2.reshape(-1, 4)).all(axis=1).mean())
-    res["raytraced"] = {"call": f"rn_render(RN_RAYTRACED) cornell-box.obj {W2}x{H2} cam_z=4",
-                        "reference_cpu_s": round(t_ref2, 4), "gpu_s": round(t_gpu2, 5),
-                        "speedup": round(t_ref2 / t_gpu2, 1), "identical_pixels": round(same, 5)}
+    res["raytraced"] = {"call": c2["call"], "gpu_s": round(t_gpu2, 5),
+                        "identical_to_reference_image": _md5(img) == c2["reference_rgba_md5"],
+                        "reference_cpu_s_container": c2["reference_s"],
+                        "reference_timing": f"measured in the build container ({cal['container_threads']} threads; "
+                                            "no CPU port of this renderer)"}
     ours.close()
     return res
 
@@ -478,8 +489,9 @@ def primary_parity(rn):
 def estimators_bench(rn):
     """Photon mapping and bidirectional path tracing on the Cornell asset at
     the reference CLI's defaults (render -m photon: 640x480, 200000 photons,
-    scale 0.35, cam_z 4), beside the compiled reference (oracle/_ref) when it
-    is present."""
+    scale 0.35, cam_z 4), beside the reference's CPU time for the same calls
+    as measured in the build container (tests/golden/reference_timing.json);
+    the compiled reference itself does not run here."""
     from oracle import oracle as O
     asset = os.path.join(ROOT, "tests", "golden", "assets", "cornell-box.obj")
     W, H, nph, cz = 640, 480, 200000, 4.0
@@ -507,23 +519,18 @@ def estimators_bench(rn):
     t_bd64 = best(lambda: sc.render_bidirectional(W, H, cz, 64, 40.0), reps=1)
     res["bidirectional"] = {"workload": f"cornell-box.obj {W}x{H}, {spp} samples, exposure 40",
                             "gpu_s": round(t_bd, 5), "gpu_64spp_s": round(t_bd64, 5)}
-    ref = O.ref()
-    if ref is not None:
-        buf = np.empty(W * H * 4, np.uint8)
-        t0 = time.perf_counter()
-        ref.ref_render_photon(asset.encode(), 0.35, W, H, cz, nph, 0, buf)
-        t_ref = time.perf_counter() - t0
+    cal = reference_calibration()
+    if cal is not None:
+        # pass 2 on the GPU fed the reference's photons (the oracle's pass 1
+        # with the reference's std::mt19937 stream): the reference's image
         cs = O.CompatArrays.from_scene(sc)
         mine = sc.render_photon(rn.PhotonMap.from_photons(*O.photon_trace(cs, nph, O.RNG_MT)), W, H, cz).to_numpy()
-        same = float((mine.reshape(-1, 4) == buf.reshape(-1, 4)).all(axis=1).mean())
-        res["photon"].update({"reference_cpu_s": round(t_ref, 3), "reference_threads": os.cpu_count(),
-                              "speedup": round(t_ref / (t_trace + t_render), 1),
-                              "identical_pixels_with_reference_photons": round(same, 5)})
-        buf = np.empty(W * H * 4, np.uint8)
-        t0 = time.perf_counter()
-        ref.ref_render_bidirectional(asset.encode(), 0.35, W, H, cz, spp, 40.0, buf)
-        t_ref = time.perf_counter() - t0
-        res["bidirectional"].update({"reference_cpu_s": round(t_ref, 3), "speedup": round(t_ref / t_bd, 1)})
+        cp, cb = cal["photon"], cal["bidirectional"]
+        note = f"measured in the build container ({cal['container_threads']} threads)"
+        res["photon"].update({"identical_to_reference_image_with_reference_photons":
+                              _md5(mine) == cp["reference_rgba_md5"],
+                              "reference_cpu_s_container": cp["reference_s"], "reference_timing": note})
+        res["bidirectional"].update({"reference_cpu_s_container": cb["reference_s"], "reference_timing": note})
     sc.close()
     return res
 

@@ -51,22 +51,31 @@ class RenderMode(IntEnum):
 
     @classmethod
     def coerce(cls, value: Union["RenderMode", int, str]) -> "RenderMode":
-        """Coerce an int, a case-insensitive name or a member
-        (bindings/python/rednoise/__init__.py:52-74 semantics)."""
+        """The member named by a member, its integer value or its name.
+
+        Names are looked up in a table of folded spellings: case, surrounding
+        blanks and the separators '-', '_' and ' ' do not matter, so
+        "path_traced" and "Path-Traced" both give PATHTRACED. Unknown names
+        and values raise ValueError; a bool (an int subclass) and any other
+        type raise TypeError. The accepted inputs and exception types are
+        those of bindings/python/rednoise/__init__.py:51-73."""
         if isinstance(value, cls):
             return value
-        if isinstance(value, bool):
-            raise TypeError("render mode must not be a bool")
-        if isinstance(value, int):
-            return cls(value)
         if isinstance(value, str):
-            key = value.strip().upper().replace("-", "").replace("_", "").replace(" ", "")
-            for member in cls:
-                if member.name.replace("_", "") == key:
-                    return member
-            valid = ", ".join(m.name.lower() for m in cls)
-            raise ValueError(f"unknown render mode {value!r}; expected one of: {valid}")
-        raise TypeError(f"render mode must be RenderMode, int or str, not {type(value).__name__}")
+            member = _MODES_BY_FOLDED_NAME.get(_fold_mode_name(value))
+            if member is None:
+                raise ValueError(f"no render mode is called {value!r} (known: {', '.join(_MODES_BY_FOLDED_NAME)})")
+            return member
+        if isinstance(value, int) and not isinstance(value, bool):
+            return cls(value)
+        raise TypeError(f"a render mode is a RenderMode, an int or a name; got {type(value).__name__}")
+
+
+def _fold_mode_name(name: str) -> str:
+    return "".join(ch for ch in name.strip().lower() if ch not in "-_ ")
+
+
+_MODES_BY_FOLDED_NAME = {_fold_mode_name(m.name): m for m in RenderMode}
 
 
 # ----------------------------------------------------------------- structs --
@@ -396,15 +405,33 @@ class Image:
         return np.frombuffer(self._rgba, dtype=np.uint8).reshape(self._height, self._width, 4)
 
     def save(self, path: Union[str, os.PathLike]) -> None:
-        """Write a PNG through ``rn_save_png``."""
+        """Write the image. A name ending in .png (or with no extension) is
+        written by the library's own PNG encoder (``rn_save_png``); any other
+        extension is handed to Pillow when it is installed, as the reference
+        binding does, and raises ValueError when it is not."""
         path = os.fspath(path)
         ext = os.path.splitext(path)[1].lower()
         if ext and ext != ".png":
-            raise ValueError(f"only .png is supported; got {ext!r}")
+            try:
+                pil = self.to_pillow()
+            except ImportError:
+                raise ValueError(f"{ext!r} needs Pillow, which is not installed; the native writer only "
+                                 "writes .png") from None
+            pil.save(path)
+            return
         lib = _load_library()
         buf = np.frombuffer(self._rgba, dtype=np.uint8)
         if lib.rn_save_png(os.fsencode(path), self._width, self._height, _ptr(buf)) != 1:
             raise RuntimeError(f"rn_save_png failed to write {path!r}")
+
+    def to_pillow(self):
+        """The image as a ``PIL.Image.Image`` in RGBA mode. Pillow is optional
+        and imported only here; ImportError says so when it is missing."""
+        try:
+            from PIL import Image as _PIL
+        except ImportError as exc:
+            raise ImportError("Image.to_pillow() needs Pillow (the PIL package)") from exc
+        return _PIL.frombytes("RGBA", (self._width, self._height), self._rgba)
 
     def __repr__(self) -> str:
         return f"<rednoise.Image {self._width}x{self._height} RGBA8>"

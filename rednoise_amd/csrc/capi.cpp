@@ -15,6 +15,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <numeric>
 
@@ -952,14 +953,28 @@ int rn_render_batch(const rn_batch_item *items, int count, const rn_batch_option
 	rn_batch_options o{};
 	if (opts)
 		o = *opts;
+	for (int i = 0; i < count; i++)
+		if (!items[i].scene || !items[i].hdr_rgb)
+			throw Error("rn_render_batch: NULL scene or output");
+	// Every distinct scene stays locked for the whole batch, as one scene is
+	// for a single render: set_materials / set_builder cannot change a scene
+	// between its upload and the end of the render. Locks are taken in
+	// address order so two batches over the same scenes cannot deadlock.
+	std::vector<rn_scene *> distinct((size_t)count);
+	for (int i = 0; i < count; i++)
+		distinct[(size_t)i] = const_cast<rn_scene *>(items[i].scene);
+	std::sort(distinct.begin(), distinct.end(), std::less<rn_scene *>());
+	distinct.erase(std::unique(distinct.begin(), distinct.end()), distinct.end());
+	std::vector<std::unique_lock<std::mutex>> scene_locks;
+	scene_locks.reserve(distinct.size());
+	for (rn_scene *sc : distinct)
+		scene_locks.emplace_back(sc->mu);
 	std::vector<WaveItem> w((size_t)count);
 	std::vector<size_t> offs((size_t)count + 1, 0);
 	double build_ms = 0.0;
 	int max_depth_bvh = 0, nodes = 0;
 	for (int i = 0; i < count; i++) {
 		const rn_batch_item &it = items[i];
-		if (!it.scene || !it.hdr_rgb)
-			throw Error("rn_render_batch: NULL scene or output");
 		rn_scene *sc = const_cast<rn_scene *>(it.scene);
 		bool had = sc->bvh_valid && sc->bvh.width == (o.bvh_width > 0 ? o.bvh_width : 8);
 		sc->ensure_bvh(o.bvh_width, c.stream);

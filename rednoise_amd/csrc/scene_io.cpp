@@ -597,6 +597,20 @@ bool load_scene(const std::string &path, HostScene &sc, std::string &err) {
 		err = path + ": truncated header";
 		return false;
 	}
+	// Bytes after the 32-byte header. Every count read below is checked
+	// against them before any array is sized, so a truncated file or a
+	// corrupt header cannot make the loader allocate what the file does not
+	// hold.
+	long long left = -1;
+	if (std::fseek(f, 0, SEEK_END) == 0) {
+		const long long end = std::ftell(f);
+		if (end >= 32 && std::fseek(f, 32, SEEK_SET) == 0)
+			left = end - 32;
+	}
+	if (left < 0) {
+		err = path + ": cannot determine the file size";
+		return false;
+	}
 	if (hdr[0] != kCacheVersion) {
 		err = path + ": unsupported scene cache version " + std::to_string(hdr[0]);
 		return false;
@@ -606,6 +620,14 @@ bool load_scene(const std::string &path, HostScene &sc, std::string &err) {
 		err = path + ": implausible triangle or texture count";
 		return false;
 	}
+	// per triangle: verts 9, albedo 3, emission 3, normals 3, vertex normals 9,
+	// texcoords 6 floats; colour 3, material 1, texture id 1 ints
+	constexpr long long kTriBytes = (9 + 3 + 3 + 3 + 9 + 6) * 4 + (3 + 1 + 1) * 4;
+	if ((long long)n * kTriBytes > left) {
+		err = path + ": truncated (the header counts more triangles than the file holds)";
+		return false;
+	}
+	left -= (long long)n * kTriBytes;
 	HostScene h;
 	h.n = (int)n;
 	CacheHash hs;
@@ -619,10 +641,19 @@ bool load_scene(const std::string &path, HostScene &sc, std::string &err) {
 		if (!ok)
 			break;
 		hs.add(wh, sizeof wh);
-		if (wh[0] < 0 || wh[1] < 0 || (int64_t)wh[0] * wh[1] > (int64_t)1 << 28) {
+		left -= 8;
+		// the OBJ loader's textures are at least 1x1 (load_ppm rejects empty
+		// images), and a texel lookup into an empty texture would read
+		// outside the texel buffer
+		if (wh[0] <= 0 || wh[1] <= 0 || (int64_t)wh[0] * wh[1] > (int64_t)1 << 28) {
 			err = path + ": implausible texture size";
 			return false;
 		}
+		if ((long long)wh[0] * wh[1] * 4 > left) {
+			err = path + ": truncated";
+			return false;
+		}
+		left -= (long long)wh[0] * wh[1] * 4;
 		h.textures[k].width = wh[0];
 		h.textures[k].height = wh[1];
 		ok = get(f, hs, h.textures[k].argb, (size_t)wh[0] * (size_t)wh[1]);

@@ -2,6 +2,7 @@
 host-side error contract, the OBJ/MTL/PPM loader against the reference's
 loadOBJ goldens, and the PNG writer."""
 import ctypes as C
+import importlib.util
 import os
 import re
 import struct
@@ -181,3 +182,28 @@ def test_photon_and_bidirectional_argument_errors_host_side(rn):
     assert lib.rn_photon_map_count(None) == 0
     assert lib.rn_photon_map_trace(None, 10, 0, 0) is None
     lib.rn_photon_map_free(None)
+
+
+def test_reference_import_names():
+    """`import rednoise` and `rednoise.torch` resolve to this library, so a
+    script written against the reference binding needs no edit."""
+    import rednoise
+    import rednoise.torch as rt
+    import rednoise_amd
+    from rednoise_amd.differentiable import PathReplayRender
+    assert rednoise.__all__[:4] == ["RenderMode", "Scene", "Image", "version"]
+    for name in ("RenderMode", "Scene", "Image", "version", "Camera", "RenderParams", "render_batch"):
+        assert getattr(rednoise, name) is getattr(rednoise_amd, name)
+    assert rt.PathReplayRender is PathReplayRender
+    assert rednoise.RenderMode.coerce("Path-Traced") is rednoise.RenderMode.PATHTRACED
+    assert rednoise.RenderMode.coerce(2) is rednoise.RenderMode.RAYTRACED
+    with pytest.raises(ValueError):
+        rednoise.RenderMode.coerce(9)
+    with pytest.raises(TypeError):
+        rednoise.RenderMode.coerce(3.0)
+    if importlib.util.find_spec("PIL") is None:
+        img = rednoise.Image(bytes(16), 2, 2)
+        with pytest.raises(ValueError, match="Pillow"):
+            img.save("x.jpg")
+        with pytest.raises(ImportError):
+            img.to_pillow()

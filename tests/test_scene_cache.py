@@ -105,6 +105,31 @@ def test_corrupt_truncated_foreign(rn, tmp_path):
         rn.Scene.load(tmp_path / "obj")
     with pytest.raises(FileNotFoundError):
         rn.Scene.load(tmp_path / "missing")
+    # a header that claims 2^29 triangles over a few bytes of payload is
+    # rejected from the file size, before any array is sized for it
+    huge = bytearray(raw[:32]) + bytearray(64)
+    struct.pack_into("<I", huge, 12, 1 << 29)
+    (tmp_path / "huge").write_bytes(bytes(huge))
+    with pytest.raises(RuntimeError, match="truncated"):
+        rn.Scene.load(tmp_path / "huge")
+
+
+@pytest.mark.parametrize("w,h", [(0, 395), (480, 0), (-1, 4)])
+def test_empty_texture_rejected(rn, tmp_path, w, h):
+    """A texture record of zero (or negative) width or height is refused: the
+    texel lookup of a textured triangle needs at least one texel."""
+    a = rn.Scene.load_obj(os.path.join(ASSETS, "textured-quad.obj"), 0.35)
+    p = tmp_path / "q.rnscene"
+    a.save(p)
+    raw = bytearray(p.read_bytes())
+    n, ntex = struct.unpack_from("<2I", raw, 12)
+    assert ntex == 1
+    off = 32 + n * (9 + 3 + 3 + 3 + 9 + 3 + 1 + 1 + 6) * 4
+    assert struct.unpack_from("<2i", raw, off) == (480, 395)
+    struct.pack_into("<2i", raw, off, w, h)
+    (tmp_path / "e").write_bytes(bytes(raw))
+    with pytest.raises(RuntimeError, match="texture size"):
+        rn.Scene.load(tmp_path / "e")
 
 
 @pytest.mark.parametrize("key,asset,scale", [("cornell", "cornell-box.obj", 0.35), ("quad", "textured-quad.obj", 0.35),
