@@ -122,6 +122,28 @@ int rn_render_ex(const rn_scene *scene, const rn_camera *camera, const rn_render
 int rn_render_ex_device(const rn_scene *scene, const rn_camera *camera, const rn_render_params *params,
                         const float *d_albedo, const float *d_emission, float *d_hdr_rgb, void *stream);
 
+/* Allocation-free form of rn_render_ex_device. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (serving
+ * loops, harnesses that forbid hipMalloc, and -- with the megakernel, which
+ * only enqueues work -- code under HIP graph capture). It calls no hipMalloc,
+ * hipFree, stream or event creation: all of its work memory (counters and the
+ * wavefront's queues, control block and scene table) lives in one device
+ * array of about 86 MB per GPU that the library reserves when it is loaded
+ * and holds until the process ends.
+ *   - The scene must already hold its device copy and a BVH of
+ *     params->bvh_width (call rn_scene_build first); otherwise it returns 0.
+ *   - RN_INTEGRATOR_MEGAKERNEL enqueues on `stream` and returns.
+ *     RN_INTEGRATOR_WAVEFRONT keeps at most RN_NOALLOC_MAX_QUEUE_PATHS paths
+ *     in flight (any image size; larger frames refill the queue) and
+ *     synchronises `stream` before it returns; queue_paths above that
+ *     maximum, RN_FLAG_USE_GRAPH and RN_FLAG_TIME_STAGES are rejected.
+ *   - Every call on one GPU shares that GPU's array: calls whose GPU work
+ *     overlaps (on different streams, from different threads) race on it.
+ * The image equals rn_render_ex_device's (same Philox streams). */
+#define RN_NOALLOC_MAX_QUEUE_PATHS 524288
+int rn_render_ex_device_noalloc(const rn_scene *scene, const rn_camera *camera, const rn_render_params *params,
+                                const float *d_albedo, const float *d_emission, float *d_hdr_rgb, void *stream);
+
 /* Path-replay adjoint of rn_render_ex: given dL/dI (width*height*3, the loss
  * gradient with respect to each pixel's mean radiance) writes dL/d albedo and
  * dL/d emission (3*n each; either may be NULL). Re-traces every path from its

@@ -192,6 +192,7 @@ def _configure(lib: C.CDLL) -> None:
         "rn_scene_get_texture": ([vp, i, vp, vp, vp], i),
         "rn_render_ex": ([vp, C.POINTER(Camera), C.POINTER(RenderParams), vp], i),
         "rn_render_ex_device": ([vp, C.POINTER(Camera), C.POINTER(RenderParams), vp, vp, vp, vp], i),
+        "rn_render_ex_device_noalloc": ([vp, C.POINTER(Camera), C.POINTER(RenderParams), vp, vp, vp, vp], i),
         "rn_adjoint": ([vp, C.POINTER(Camera), C.POINTER(RenderParams), vp, vp, vp], i),
         "rn_adjoint_device": ([vp, C.POINTER(Camera), C.POINTER(RenderParams), vp, vp, vp, vp, vp, vp], i),
         "rn_render_batch": ([C.POINTER(BatchItem), i, C.POINTER(BatchOptions)], i),

@@ -786,6 +786,112 @@ int rn_render_ex_device(const rn_scene *scene, const rn_camera *camera, const rn
 	RN_CATCH(0)
 }
 
+} // extern "C"
+
+// ------------------------------------------------ allocation-free render --
+//
+// rn_render_ex_device_noalloc keeps all of its work memory in `scratch`, one
+// file-scope device array per GPU, reserved when the library's code object
+// is loaded and held until the process ends: the integrators' counters, then
+// the wavefront's queues for up to RN_NOALLOC_MAX_QUEUE_PATHS paths in flight,
+// its control block and its one-entry scene table. In host code `scratch`
+// names a host-side stand-in, never the device memory: the device address is
+// looked up once per GPU with hipGetSymbolAddress (with that GPU current)
+// and only that address is handed to kernels and fills.
+constexpr size_t kCountersBytes = (sizeof(DevCounters) + 255) / 256 * 256;
+constexpr size_t kNoAllocPaths = RN_NOALLOC_MAX_QUEUE_PATHS;
+// counters + 164 bytes per path + control block, scene table and alignment
+// (86 MB in all)
+constexpr size_t kNoAllocBytes = kCountersBytes + kNoAllocPaths * (10 * 16 + 4) + 16384;
+static_assert(sizeof(DevCounters) <= kCountersBytes, "counter block");
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;     // guards g_scratch_ptr
+void *g_scratch_ptr[kMaxDevices] = {};
+std::mutex g_noalloc_mu;     // one allocation-free render at a time (host side)
+
+// Device address of `scratch` on the current GPU.
+void *scratch_on_current_device() {
+	int dev = -1;
+	RN_HIP(hipGetDevice(&dev));
+	if (dev < 0 || dev >= kMaxDevices)
+		throw Error("rn_render_ex_device_noalloc: device index " + std::to_string(dev) + " out of range");
+	std::lock_guard<std::mutex> lk(g_scratch_mu);
+	if (!g_scratch_ptr[dev]) {
+		void *p = nullptr;
+		RN_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+		if (!p)
+			throw Error("rn_render_ex_device_noalloc: no device address for the work memory");
+		g_scratch_ptr[dev] = p;
+	}
+	return g_scratch_ptr[dev];
+}
+
+} // namespace
+
+extern "C" {
+
+int rn_render_ex_device_noalloc(const rn_scene *scene, const rn_camera *camera, const rn_render_params *params,
+                                const float *d_albedo, const float *d_emission, float *d_hdr_rgb, void *stream) {
+	RN_TRY
+	t_error.clear();
+	if (!scene || !camera || !params || !d_hdr_rgb)
+		throw Error("rn_render_ex_device_noalloc: NULL scene, camera, params or output");
+	if (params->flags & (RN_FLAG_USE_GRAPH | RN_FLAG_TIME_STAGES))
+		throw Error("rn_render_ex_device_noalloc: RN_FLAG_USE_GRAPH and RN_FLAG_TIME_STAGES are not available "
+		            "(both need memory of their own)");
+	const bool wave = params->integrator == RN_INTEGRATOR_WAVEFRONT;
+	if (params->integrator != RN_INTEGRATOR_WAVEFRONT && params->integrator != RN_INTEGRATOR_MEGAKERNEL)
+		throw Error("rn_render_ex_device_noalloc: unknown integrator");
+	if (wave && params->queue_paths > (int)kNoAllocPaths)
+		throw Error("rn_render_ex_device_noalloc: queue_paths " + std::to_string(params->queue_paths) +
+		            " exceeds the maximum of " + std::to_string(kNoAllocPaths) + " paths in flight");
+	auto t0 = std::chrono::steady_clock::now();
+	std::lock_guard<std::mutex> nl(g_noalloc_mu);
+	rn_scene *sc = const_cast<rn_scene *>(scene);
+	std::lock_guard<std::mutex> sl(sc->mu);
+	const int width = params->bvh_width > 0 ? params->bvh_width : 8;
+	if (!sc->uploaded || !sc->bvh_valid || sc->bvh.width != width)
+		throw Error("rn_render_ex_device_noalloc: the scene has no device copy or no BVH" + std::to_string(width) +
+		            " yet; call rn_scene_build(scene, " + std::to_string(width) +
+		            ") first (this entry point neither uploads nor builds)");
+	DevParams p = make_params(*params, sc, 0);
+	DevCamera cam = make_camera(*camera, p.width, p.height);
+	hipStream_t s = static_cast<hipStream_t>(stream);
+	char *base = static_cast<char *>(scratch_on_current_device());
+	DevCounters *cnt = reinterpret_cast<DevCounters *>(base);
+	fill_zero(cnt, sizeof(DevCounters), s);
+	DevScene ds = sc->dev(d_albedo, d_emission);
+	WaveStats ws{};
+	if (wave) {
+		WaveItem item{ds, cam, p, d_hdr_rgb};
+		WaveConfig cfg;
+		cfg.queue_paths = params->queue_paths;
+		cfg.block = params->block_size;
+		WaveScratch mem;
+		mem.base = base + kCountersBytes;
+		mem.bytes = kNoAllocBytes - kCountersBytes;
+		if (wave_scratch_capacity(mem.bytes, 1) < kNoAllocPaths)
+			throw Error("rn_render_ex_device_noalloc: internal error, work memory smaller than its queue");
+		wavefront_render(&item, 1, cfg, cnt, s, ws, &mem);
+	} else {
+		launch_mega(ds, cam, p, d_hdr_rgb, cnt, (params->flags & RN_FLAG_COUNT_TRAVERSAL) != 0, s);
+	}
+	std::memset(&t_stats, 0, sizeof t_stats);
+	t_stats.total_ms = ms_since(t0);
+	t_stats.paths = (uint64_t)p.width * p.height * p.spp;
+	t_stats.iterations = ws.iterations;
+	t_stats.trace_kernel_launches = wave ? ws.extend_launches : 1;
+	t_stats.bvh_nodes = sc->bvh.n_nodes;
+	t_stats.bvh_depth = sc->bvh.depth;
+	return 1;
+	RN_CATCH(0)
+}
+
 rn_accum *rn_accum_create(int width, int height) {
 	RN_TRY
 	t_error.clear();

@@ -27,10 +27,23 @@ struct WaveStats {
 	uint64_t extend_launches = 0, iterations = 0;
 };
 
+// Device memory handed in by the caller in place of the library's own
+// queues: paths in flight, the control block and the scene table are carved
+// from [base, base + bytes). With it the render allocates nothing (no graph,
+// no stage timing; the done flag is read into host stack memory).
+struct WaveScratch {
+	void *base = nullptr;
+	size_t bytes = 0;
+};
+// Bytes of caller memory a render of `count` scenes with `capacity` paths in
+// flight needs, and the largest capacity that fits in `bytes`.
+size_t wave_scratch_bytes(unsigned capacity, int count);
+unsigned wave_scratch_capacity(size_t bytes, int count);
+
 // Renders every item (outputs zeroed, accumulated and divided by spp).
 // Counters receive closest-hit rays, shadow rays and paths.
 void wavefront_render(const WaveItem *items, int count, const WaveConfig &cfg, DevCounters *cnt, hipStream_t s,
-                      WaveStats &st);
+                      WaveStats &st, const WaveScratch *scratch = nullptr);
 
 void fill_zero(void *p, size_t bytes, hipStream_t s);
 

@@ -551,14 +551,48 @@ void fill_zero(void *p, size_t bytes, hipStream_t s) {
 }
 
 void wavefront_render_on(const WaveItem *items, int count, const WaveConfig &cfg, DevCounters *cnt, hipStream_t s,
-                         WaveStats &st);
+                         WaveStats &st, const WaveScratch *ext);
+
+// Per path in flight: path state double buffered (6 float4), closest-hit
+// record (float4 + int) and shadow ray (3 float4).
+constexpr size_t kEntryBytes = 10 * sizeof(float4) + sizeof(int);
+constexpr size_t kScratchAlign = 256;
+
+static size_t align_up(size_t x) { return (x + kScratchAlign - 1) / kScratchAlign * kScratchAlign; }
+
+static size_t scratch_fixed_bytes(int count) {
+	return align_up(sizeof(WCtl)) + align_up((size_t)count * sizeof(WScene));
+}
+
+size_t wave_scratch_bytes(unsigned capacity, int count) {
+	const size_t c = capacity;
+	return align_up(c * 6 * sizeof(float4)) + align_up(c * (sizeof(float4) + sizeof(int))) +
+	       align_up(c * 3 * sizeof(float4)) + scratch_fixed_bytes(count);
+}
+
+unsigned wave_scratch_capacity(size_t bytes, int count) {
+	const size_t fixed = scratch_fixed_bytes(count) + 3 * kScratchAlign; // alignment slack of the three arrays
+	if (bytes <= fixed)
+		return 0;
+	size_t cap = (bytes - fixed) / kEntryBytes;
+	cap = std::min<size_t>(cap, 1u << 30);
+	while (cap > 0 && wave_scratch_bytes((unsigned)cap, count) > bytes)
+		cap--;
+	return (unsigned)cap;
+}
 
 // The legacy default stream cannot be captured: a graph requested there runs
 // on a library stream ordered after, and before, the caller's work.
 void wavefront_render(const WaveItem *items, int count, const WaveConfig &cfg, DevCounters *cnt, hipStream_t s,
-                      WaveStats &st) {
+                      WaveStats &st, const WaveScratch *scratch) {
+	if (scratch) {
+		if (cfg.use_graph || cfg.time_stages)
+			throw Error("wavefront: a render in caller memory takes no graph and no stage timing");
+		wavefront_render_on(items, count, cfg, cnt, s, st, scratch);
+		return;
+	}
 	if (!(cfg.use_graph && s == nullptr)) {
-		wavefront_render_on(items, count, cfg, cnt, s, st);
+		wavefront_render_on(items, count, cfg, cnt, s, st, nullptr);
 		return;
 	}
 	WaveWork &ww = work();
@@ -569,13 +603,13 @@ void wavefront_render(const WaveItem *items, int count, const WaveConfig &cfg, D
 	}
 	RN_HIP(hipEventRecord(ww.join[0], s));
 	RN_HIP(hipStreamWaitEvent(ww.capture_stream, ww.join[0], 0));
-	wavefront_render_on(items, count, cfg, cnt, ww.capture_stream, st);
+	wavefront_render_on(items, count, cfg, cnt, ww.capture_stream, st, nullptr);
 	RN_HIP(hipEventRecord(ww.join[1], ww.capture_stream));
 	RN_HIP(hipStreamWaitEvent(s, ww.join[1], 0));
 }
 
 void wavefront_render_on(const WaveItem *items, int count, const WaveConfig &cfg, DevCounters *cnt, hipStream_t s,
-                         WaveStats &st) {
+                         WaveStats &st, const WaveScratch *ext) {
 	if (count <= 0 || count > 65535)
 		throw Error("wavefront: batch size must be in [1, 65535]");
 	std::vector<WScene> hs((size_t)count);
@@ -602,11 +636,18 @@ void wavefront_render_on(const WaveItem *items, int count, const WaveConfig &cfg
 	// a tail in which its waves drain, so fewer, larger launches win: at
 	// 1,048,588 triangles / 1080p / 64 spp a frame took 142 ms with 2M paths
 	// in flight, 81 ms with 16M and 71 ms with the whole frame (133M) in
-	// flight. By default up to 2^27 paths (160 bytes each, 21 GB) are kept,
+	// flight. By default up to 2^27 paths (164 bytes each, 22 GB) are kept,
 	// but no more than a quarter of the free device memory.
-	constexpr size_t kEntryBytes = 10 * sizeof(float4) + sizeof(int); // path state 2 x 3, shadow ray 3, hit
 	unsigned long long want;
-	if (cfg.queue_paths > 0) {
+	const unsigned ext_cap = ext ? wave_scratch_capacity(ext->bytes, count) : 0u;
+	if (ext) {
+		if (ext_cap < 64)
+			throw Error("wavefront: the caller's work memory holds fewer than 64 paths");
+		if (cfg.queue_paths > 0 && (unsigned long long)cfg.queue_paths > ext_cap)
+			throw Error("wavefront: queue_paths " + std::to_string(cfg.queue_paths) +
+			            " exceeds the caller's work memory (" + std::to_string(ext_cap) + " paths)");
+		want = cfg.queue_paths > 0 ? (unsigned long long)cfg.queue_paths : ext_cap;
+	} else if (cfg.queue_paths > 0) {
 		want = std::min<unsigned long long>((unsigned long long)cfg.queue_paths, 1ull << 30);
 	} else {
 		size_t freeb = 0, totb = 0;
@@ -617,22 +658,33 @@ void wavefront_render_on(const WaveItem *items, int count, const WaveConfig &cfg
 	}
 	const unsigned cap = (unsigned)std::max<unsigned long long>(std::min(total, want), 64ull);
 	WQ q;
-	float4 *st4 = static_cast<float4 *>(ww.state.reserve((size_t)cap * 6 * sizeof(float4)));
+	// carve the queues from the caller's memory, or reserve the library's own
+	char *carve = ext ? static_cast<char *>(ext->base) : nullptr;
+	auto take = [&](DevBuf &own, size_t bytes) -> void * {
+		if (!ext)
+			return own.reserve(bytes);
+		void *p = carve;
+		carve += align_up(bytes);
+		return p;
+	};
+	float4 *st4 = static_cast<float4 *>(take(ww.state, (size_t)cap * 6 * sizeof(float4)));
 	q.o[0] = st4;
 	q.d[0] = st4 + cap;
 	q.b[0] = st4 + 2 * (size_t)cap;
 	q.o[1] = st4 + 3 * (size_t)cap;
 	q.d[1] = st4 + 4 * (size_t)cap;
 	q.b[1] = st4 + 5 * (size_t)cap;
-	float4 *hit4 = static_cast<float4 *>(ww.hit.reserve((size_t)cap * (sizeof(float4) + sizeof(int))));
+	float4 *hit4 = static_cast<float4 *>(take(ww.hit, (size_t)cap * (sizeof(float4) + sizeof(int))));
 	q.hitn = hit4;
 	q.hitp = reinterpret_cast<int *>(hit4 + cap);
-	float4 *sh4 = static_cast<float4 *>(ww.shadow.reserve((size_t)cap * 3 * sizeof(float4)));
+	float4 *sh4 = static_cast<float4 *>(take(ww.shadow, (size_t)cap * 3 * sizeof(float4)));
 	q.so = sh4;
 	q.sd = sh4 + cap;
 	q.sc = sh4 + 2 * (size_t)cap;
-	q.ctl = static_cast<WCtl *>(ww.ctl.reserve(sizeof(WCtl)));
-	WScene *dsc = static_cast<WScene *>(ww.scenes.reserve(hs.size() * sizeof(WScene)));
+	q.ctl = static_cast<WCtl *>(take(ww.ctl, sizeof(WCtl)));
+	WScene *dsc = static_cast<WScene *>(take(ww.scenes, hs.size() * sizeof(WScene)));
+	if (ext && (size_t)(carve - static_cast<char *>(ext->base)) > ext->bytes)
+		throw Error("wavefront: internal error, caller work memory overrun"); // excluded by wave_scratch_capacity
 	RN_HIP(hipMemcpyAsync(dsc, hs.data(), hs.size() * sizeof(WScene), hipMemcpyHostToDevice, s));
 	q.scenes = dsc;
 	q.nscenes = count;
@@ -642,10 +694,9 @@ void wavefront_render_on(const WaveItem *items, int count, const WaveConfig &cfg
 	q.capacity = cap;
 	q.cnt = cnt;
 
-	hipDeviceProp_t prop;
-	int dev = 0;
+	int dev = 0, n_cu = 0;
 	RN_HIP(hipGetDevice(&dev));
-	RN_HIP(hipGetDeviceProperties(&prop, dev));
+	RN_HIP(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
 	bool deep = max_depth - 1 > kLdsDepth;
 
 	// Stage timing: events bracket each launch; pairs are summed at the end.
@@ -663,11 +714,11 @@ void wavefront_render_on(const WaveItem *items, int count, const WaveConfig &cfg
 	// Trace kernels: enough resident waves to fill every SIMD; each wave
 	// pulls chunks until the queue is exhausted. Shade and generate: plain
 	// grid-stride launches.
-	unsigned grid = (unsigned)std::max(1, prop.multiProcessorCount) * (bs == 128 ? 16u : 8u);
+	unsigned grid = (unsigned)std::max(1, n_cu) * (bs == 128 ? 16u : 8u);
 	const unsigned chunks = (cap + kChunkMin - 1) / kChunkMin, waves_per_block = (unsigned)bs / 64u;
 	grid = std::min<unsigned>(grid, (chunks + waves_per_block - 1) / waves_per_block);
-	const unsigned sgrid = std::min<unsigned>((unsigned)std::max(1, prop.multiProcessorCount) * 8u, (cap + 255) / 256);
-	const unsigned shgrid = std::min<unsigned>((unsigned)std::max(1, prop.multiProcessorCount) * (2048u / kShadeBlock),
+	const unsigned sgrid = std::min<unsigned>((unsigned)std::max(1, n_cu) * 8u, (cap + 255) / 256);
+	const unsigned shgrid = std::min<unsigned>((unsigned)std::max(1, n_cu) * (2048u / kShadeBlock),
 	                                           (cap + kShadeBlock - 1) / kShadeBlock);
 	auto iteration = [&](int cur) {
 		mark(1, true);
@@ -712,8 +763,12 @@ void wavefront_render_on(const WaveItem *items, int count, const WaveConfig &cfg
 	k_init<<<1, 1, 0, s>>>(q.ctl, total);
 	k_advance<<<1, 1, 0, s>>>(q, 1);
 	RN_HIP(hipGetLastError());
-	unsigned *flag = nullptr;
-	RN_HIP(hipHostMalloc(reinterpret_cast<void **>(&flag), sizeof(unsigned) * 2, hipHostMallocDefault));
+	// the done flag is read into pinned memory, or into the stack when the
+	// render must allocate nothing
+	unsigned stack_flag[2] = {0u, 0u};
+	unsigned *flag = stack_flag;
+	if (!ext)
+		RN_HIP(hipHostMalloc(reinterpret_cast<void **>(&flag), sizeof(unsigned) * 2, hipHostMallocDefault));
 	int per = cfg.iters_per_graph > 0 ? cfg.iters_per_graph : 8;
 	per += per & 1;
 	hipGraphExec_t exec = nullptr;
@@ -767,14 +822,16 @@ void wavefront_render_on(const WaveItem *items, int count, const WaveConfig &cfg
 			(void)hipGraphExecDestroy(exec);
 		if (graph)
 			(void)hipGraphDestroy(graph);
-		(void)hipHostFree(flag);
+		if (flag != stack_flag)
+			(void)hipHostFree(flag);
 		throw;
 	}
 	if (exec)
 		RN_HIP(hipGraphExecDestroy(exec));
 	if (graph)
 		RN_HIP(hipGraphDestroy(graph));
-	RN_HIP(hipHostFree(flag));
+	if (flag != stack_flag)
+		RN_HIP(hipHostFree(flag));
 	for (int i = 0; i < count; i++) {
 		size_t n = (size_t)items[i].params.width * items[i].params.height * 3;
 		k_resolve<<<(unsigned)std::min<size_t>(2048, (n + 255) / 256), 256, 0, s>>>(items[i].out, n,
