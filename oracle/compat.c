@@ -201,6 +201,21 @@ static int cbuild(cbvh *b, int start, int count) {
 	return me;
 }
 
+int or_compat_all_hits(const float *verts, int n, const float o[3], const float d[3], int *prims, float *ts, int cap) {
+	int k = 0;
+	for (int i = 0; i < n; i++) {
+		float tt, uu, vv;
+		if (or_compat_intersect_tri(o, d, verts + 9 * i, verts + 9 * i + 3, verts + 9 * i + 6, &tt, &uu, &vv)) {
+			if (k < cap) {
+				prims[k] = i;
+				ts[k] = tt;
+			}
+			k++;
+		}
+	}
+	return k;
+}
+
 void *or_compat_bvh_build(const float *verts, int n) {
 	cbvh *b = (cbvh *)calloc(1, sizeof(cbvh));
 	b->verts = verts;

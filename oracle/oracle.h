@@ -186,6 +186,13 @@ void or_phys_render_list_f32(const void *s, const or_cam *cam, const or_params *
 void or_phys_adjoint_f64(const void *s, const or_cam *cam, const or_params *p, const float *dl_dimage, int nthreads,
                          double *g_albedo, double *g_emission);
 /* Closest hit in the given precision (brute force when brute != 0). */
+int or_phys_all_hits_f32(const void *s, const float o[3], const float d[3], float tmax, int *prims, float *ts,
+                         int cap);
+int or_phys_all_hits_f64(const void *s, const double o[3], const double d[3], double tmax, int *prims, double *ts,
+                         int cap);
+/* Every triangle the reference test (src/Geometry.cpp:11-25) accepts on
+ * (o, d), by brute force: ids and t (at most cap each); returns the count. */
+int or_compat_all_hits(const float *verts, int n, const float o[3], const float d[3], int *prims, float *ts, int cap);
 int or_phys_closest_f64(const void *s, const double o[3], const double d[3], double tmax, int brute, double *t,
                         double *u, double *v);
 int or_phys_closest_f32(const void *s, const float o[3], const float d[3], float tmax, int brute, float *t, float *u,

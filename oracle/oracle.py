@@ -111,6 +111,10 @@ def lib():
         L.or_phys_closest_f64.restype = C.c_int
         L.or_phys_closest_f32.argtypes = [C.c_void_p, f32p, f32p, C.c_float, C.c_int] + [C.POINTER(C.c_float)] * 3
         L.or_phys_closest_f32.restype = C.c_int
+        L.or_phys_all_hits_f32.argtypes = [C.c_void_p, f32p, f32p, C.c_float, i32p, f32p, C.c_int]
+        L.or_phys_all_hits_f32.restype = C.c_int
+        L.or_compat_all_hits.argtypes = [f32p, C.c_int, f32p, f32p, i32p, f32p, C.c_int]
+        L.or_compat_all_hits.restype = C.c_int
         _lib = L
     return _lib
 
@@ -277,6 +281,33 @@ def compat_primary_rays(w: int, h: int, cam_z: float):
             o[y * w + x] = oo
             d[y * w + x] = dd
     return o, d
+
+
+def all_hits(verts, o, d, test="watertight", tmax=float("inf"), cap=64, phys=None):
+    """Every triangle the watertight test (or, with test="compat", the
+    reference's test) accepts on each ray, by brute force: a list of
+    (prims, ts) arrays per ray. `phys` reuses a prepared FP32 PhysScene."""
+    L = lib()
+    verts = np.ascontiguousarray(verts, dtype=np.float32).reshape(-1)
+    o = np.ascontiguousarray(o, dtype=np.float32).reshape(-1, 3)
+    d = np.ascontiguousarray(d, dtype=np.float32).reshape(-1, 3)
+    own = None
+    if test == "watertight" and phys is None:
+        own = phys = PhysScene(verts, np.zeros(verts.size // 3, np.float32), None, "f32")
+    prims = np.empty(cap, np.int32)
+    ts = np.empty(cap, np.float32)
+    out = []
+    for i in range(len(o)):
+        oi, di = np.ascontiguousarray(o[i]), np.ascontiguousarray(d[i])
+        if test == "watertight":
+            k = L.or_phys_all_hits_f32(phys.h, oi, di, float(tmax), prims, ts, cap)
+        else:
+            k = L.or_compat_all_hits(verts, verts.size // 9, oi, di, prims, ts, cap)
+        assert k <= cap, "more hits than the buffer holds"
+        out.append((prims[:k].copy(), ts[:k].copy()))
+    if own is not None:
+        own.close()
+    return out
 
 
 def camera_basis(eye, target, up, vfov, w, h) -> Cam:

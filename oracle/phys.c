@@ -75,6 +75,10 @@ void or_phys_adjoint_f64(const void *sp, const or_cam *cam, const or_params *p, 
 		tid = omp_get_thread_num();
 #endif
 		double *lga = ga + (size_t)tid * n3, *lge = ge + (size_t)tid * n3;
+		/* every term below is a multiple of this pixel's weight: a zero
+		 * weight contributes exactly nothing, so its paths are not traced */
+		if (dl_dimage[3 * i] == 0.0f && dl_dimage[3 * i + 1] == 0.0f && dl_dimage[3 * i + 2] == 0.0f)
+			continue;
 		int x = i % p->width, y = i / p->width;
 		V_f64 wgt = vmk_f64(dl_dimage[3 * i] / p->spp, dl_dimage[3 * i + 1] / p->spp, dl_dimage[3 * i + 2] / p->spp);
 		Vtx_f64 rec[64];

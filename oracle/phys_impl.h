@@ -275,8 +275,21 @@ static int SFX(wtest)(const SFX(WRay) * w, VT v0, VT v1, VT v2, R tmax, R *tt, R
 		U = (R)(cxby - cybx);
 		Vv = (R)(axcy - aycx);
 		W = (R)(bxay - byax);
+		/* An edge function still exactly zero takes the sign it would have
+		 * with the ray's point moved by (e1, e2), 0 < e2 << e1: the sign of
+		 * its x derivative, else of its y derivative, kept as the sign of the
+		 * zero. The two triangles of a shared edge see opposite derivatives,
+		 * so exactly one keeps the point; around a vertex exactly one
+		 * triangle of the fan keeps it. */
+		if (U == (R)0)
+			U = (R)copysign(0.0, (double)(Cy != By ? Cy - By : Bx - Cx));
+		if (Vv == (R)0)
+			Vv = (R)copysign(0.0, (double)(Ay != Cy ? Ay - Cy : Cx - Ax));
+		if (W == (R)0)
+			W = (R)copysign(0.0, (double)(By != Ay ? By - Ay : Ax - Bx));
 	}
-	if ((U < 0 || Vv < 0 || W < 0) && (U > 0 || Vv > 0 || W > 0))
+	/* a hit needs U, V and W of one sign */
+	if (!signbit(U) != !signbit(Vv) || !signbit(U) != !signbit(W))
 		return 0;
 	R det = U + Vv + W;
 	if (det == (R)0)
@@ -303,12 +316,15 @@ static int SFX(slab)(const SFX(Node) * nd, VT o, VT inv, R tmax) {
 	R te = 0, tx = tmax;
 	for (int k = 0; k < 3; k++) {
 		R oi = SFX(vget)(o, k), ii = SFX(vget)(inv, k);
+		if (ii == (R)INFINITY || ii == (R)-INFINITY) {
+			/* the ray is parallel to this slab: inside it (boundary planes
+			 * included) at every t, or never */
+			if (oi < nd->lo[k] || oi > nd->hi[k])
+				return 0;
+			continue;
+		}
 		R a = (nd->lo[k] - oi) * ii, b = (nd->hi[k] - oi) * ii;
 		R lo = a < b ? a : b, hi = a < b ? b : a;
-		if (lo != lo)
-			lo = (R)-INFINITY; /* 0 * inf on a slab plane: keep the node */
-		if (hi != hi)
-			hi = (R)INFINITY;
 		te = lo > te ? lo : te;
 		tx = hi < tx ? hi : tx;
 	}
@@ -375,6 +391,26 @@ static int SFX(any)(const ST *s, VT o, VT d, R tmax) {
 		}
 	}
 	return 0;
+}
+
+/* Every triangle the watertight test accepts on (o, d) with 0 < t < tmax, by
+ * brute force in triangle order: ids into prims, distances into ts (at most
+ * cap of each); returns the number of hits. */
+int SFX(or_phys_all_hits)(const void *p, const R o[3], const R d[3], R tmax, int *prims, R *ts, int cap) {
+	const ST *s = (const ST *)p;
+	SFX(WRay) w = SFX(wray)(SFX(vmk)(o[0], o[1], o[2]), SFX(vmk)(d[0], d[1], d[2]));
+	int n = 0;
+	for (int i = 0; i < s->n; i++) {
+		R tt, uu, vv;
+		if (SFX(wtest)(&w, s->v0[i], s->v1[i], s->v2[i], tmax, &tt, &uu, &vv)) {
+			if (n < cap) {
+				prims[n] = i;
+				ts[n] = tt;
+			}
+			n++;
+		}
+	}
+	return n;
 }
 
 int SFX(or_phys_closest)(const void *p, const R o[3], const R d[3], R tmax, int brute, R *t, R *u, R *v) {

@@ -82,13 +82,21 @@ RN_DEV TRay make_tray(float3 o, float3 d) {
 	r.o = o;
 	r.d = d;
 	// |1/d| stays below 2^40, so node slab terms scaled by 2^24 (below) stay
-	// finite for any node scale up to 2^60
+	// finite for any node scale up to 2^60. A component of magnitude at most
+	// 2^-40 (zero included, of either sign) becomes +2^-40: for the slabs the
+	// ray then drifts to the + side of every axis it does not move along, so
+	// a box whose upper face lies in the ray's plane is culled and one whose
+	// lower face does is kept. The watertight test's tie rule resolves a
+	// crossing on such a plane in favour of the triangle on the + side of
+	// both sheared axes, so the box of that triangle is always one the slab
+	// test keeps: exact crossings on box faces are found. (An infinite
+	// reciprocal would keep every box of that axis: NaN slab terms.)
 	const float tiny = 0x1p-40f;
-	float dx = fabsf(d.x) > tiny ? d.x : copysignf(tiny, d.x);
-	float dy = fabsf(d.y) > tiny ? d.y : copysignf(tiny, d.y);
-	float dz = fabsf(d.z) > tiny ? d.z : copysignf(tiny, d.z);
+	float dx = fabsf(d.x) > tiny ? d.x : tiny;
+	float dy = fabsf(d.y) > tiny ? d.y : tiny;
+	float dz = fabsf(d.z) > tiny ? d.z : tiny;
 	r.inv = f3(1.0f / dx, 1.0f / dy, 1.0f / dz);
-	r.oct = (d.x < 0.0f ? 1u : 0u) | (d.y < 0.0f ? 2u : 0u) | (d.z < 0.0f ? 4u : 0u);
+	r.oct = (dx < 0.0f ? 1u : 0u) | (dy < 0.0f ? 2u : 0u) | (dz < 0.0f ? 4u : 0u);
 	return r;
 }
 
@@ -217,11 +225,28 @@ RN_DEV bool tri_watertight(const WRay &w, float3 o, float3 v0, float3 v1, float3
 	float Cx = get3(C, w.kx) - w.sx * Cz0, Cy = get3(C, w.ky) - w.sy * Cz0;
 	float U = Cx * By - Cy * Bx, V = Ax * Cy - Ay * Cx, W = Bx * Ay - By * Ax;
 	if (U == 0.0f || V == 0.0f || W == 0.0f) {
+		// Edge functions recomputed exactly (float products are exact in
+		// double). One that is still zero takes the sign it would have if
+		// the ray's point moved by (e1, e2), 0 < e2 << e1: the sign of its x
+		// derivative, or of its y derivative when that is zero, kept as the
+		// sign of the zero. The two triangles of a shared edge see opposite
+		// derivatives, so exactly one of them keeps a point on the edge, and
+		// around a vertex exactly one triangle of a fan keeps it: a ray
+		// crossing a closed mesh at an edge or a vertex hits one triangle
+		// there, not none and not two. The derivatives' signs are exact (a
+		// difference of floats is zero only when they are equal).
 		U = (float)((double)Cx * (double)By - (double)Cy * (double)Bx);
 		V = (float)((double)Ax * (double)Cy - (double)Ay * (double)Cx);
 		W = (float)((double)Bx * (double)Ay - (double)By * (double)Ax);
+		if (U == 0.0f)
+			U = copysignf(0.0f, Cy != By ? Cy - By : Bx - Cx);
+		if (V == 0.0f)
+			V = copysignf(0.0f, Ay != Cy ? Ay - Cy : Cx - Ax);
+		if (W == 0.0f)
+			W = copysignf(0.0f, By != Ay ? By - Ay : Ax - Bx);
 	}
-	if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f))
+	// a hit needs U, V and W of one sign
+	if (((__float_as_uint(U) ^ __float_as_uint(V)) | (__float_as_uint(U) ^ __float_as_uint(W))) & 0x80000000u)
 		return false;
 	float det = U + V + W;
 	if (det == 0.0f)

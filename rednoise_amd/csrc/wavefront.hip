@@ -171,6 +171,11 @@ RN_DEV void puti(int *p, int v) {
 		*p = v;
 }
 
+// Set bits of a wave-wide mask in the lanes below this one.
+RN_DEV unsigned lanes_below(uint64_t m) {
+	return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
 // meta word: scene index (low 16 bits) | depth << 16; depth 255 = dead slot.
 constexpr uint32_t kDead = 255u;
 
@@ -256,14 +261,13 @@ RN_DEV void trace_queue(const WQ &q, int cur, uint64_t *stk, const uint8_t *octa
 	const float4 *D = QUERY == 0 ? q.d[cur] : q.sd;
 	const float4 *M = QUERY == 0 ? q.b[cur] : q.sc; // .w: meta (closest) or scene index (shadow)
 	const int lane = threadIdx.x & 63;
-	const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
 	unsigned next = 0, end = 0;
 	bool more = true, valid = false;
 	unsigned idx = 0;
 	const Bvh8Node *nodes = ONE ? q.nodes0 : nullptr;
 	const float4 *tris = ONE ? q.tris0 : nullptr;
 	Trav<SPILL> t;
-	unsigned long long nrays = 0;
+	unsigned nrays = 0; // this lane's rays (at most the queue length, below 2^31)
 	while (true) {
 		const uint64_t idle = __ballot(!valid);
 		if (more && __popcll(idle) >= kRefillMin) {
@@ -295,7 +299,8 @@ RN_DEV void trace_queue(const WQ &q, int cur, uint64_t *stk, const uint8_t *octa
 				}
 			}
 			if (next < end) {
-				const unsigned my = next + (unsigned)__popcll(idle & lt);
+				// idle lanes below this one (v_mbcnt: no lane mask held in registers)
+				const unsigned my = next + lanes_below(idle);
 				if (!valid && my < end) {
 					// the entry's loads are issued together
 					// path records stream past non-temporally; shadow records
@@ -340,7 +345,7 @@ RN_DEV void trace_queue(const WQ &q, int cur, uint64_t *stk, const uint8_t *octa
 		}
 	}
 	CounterSet *cs = &q.cnt->shard[blockIdx.x % kCounterShards];
-	wave_add_u64(QUERY == 0 ? &cs->closest : &cs->shadow, nrays);
+	wave_add_u64(QUERY == 0 ? &cs->closest : &cs->shadow, (unsigned long long)nrays);
 }
 
 // Occupancy target of the trace kernels without a spill stack, in waves per

@@ -404,3 +404,31 @@ def test_rmse_fixtures_match_their_workloads(oracle):
     se = runs.std(0, ddof=1) / 2.0 + 1e-3 * np.abs(mean) + 1e-6
     z = np.abs(mean - f0["values"][pick]) / se
     assert (z < 5).all(), z
+
+
+def test_adjoint_is_linear_in_the_pixel_weights(oracle):
+    """The adjoint is linear in dL/dI: the gradient of a weight image equals
+    the sum of the gradients of two weight images with disjoint pixel support
+    (pixels of zero weight contribute nothing and are not traced, so this
+    also checks that no pixel with a non-zero weight in any channel is
+    skipped: some weights below are zero in one or two channels only)."""
+    from rednoise_amd import scenes
+    sd = scenes.cornell_box()
+    W, H, spp, D = 16, 12, 4, 4
+    cam = oracle.camera_basis(sd.camera["eye"], sd.camera["target"], sd.camera["up"], 40.0, W, H)
+    p = oracle.params(W, H, spp, D, 100, 0.0, 5, 0, 0.05)
+    rng = np.random.default_rng(9)
+    wgt = rng.normal(size=(H, W, 3)).astype(np.float32)
+    wgt[rng.uniform(size=(H, W, 3)) < 0.3] = 0.0  # single zero channels
+    part = rng.uniform(size=(H, W)) < 0.5
+    w1 = np.where(part[..., None], wgt, 0).astype(np.float32)
+    w2 = np.where(part[..., None], 0, wgt).astype(np.float32)
+    ps = oracle.PhysScene(sd.verts, sd.albedo, sd.emission, "f64")
+    ga, ge = ps.adjoint(cam, p, wgt)
+    a1, e1 = ps.adjoint(cam, p, w1)
+    a2, e2 = ps.adjoint(cam, p, w2)
+    assert np.abs(ga).max() > 0 and np.abs(ge).max() > 0
+    assert np.allclose(ga, a1 + a2, rtol=1e-12, atol=1e-15)
+    assert np.allclose(ge, e1 + e2, rtol=1e-12, atol=1e-15)
+    z, zz = ps.adjoint(cam, p, np.zeros_like(wgt))
+    assert not z.any() and not zz.any()
