@@ -286,7 +286,8 @@ def compat_primary_rays(w: int, h: int, cam_z: float):
 def all_hits(verts, o, d, test="watertight", tmax=float("inf"), cap=64, phys=None):
     """Every triangle the watertight test (or, with test="compat", the
     reference's test) accepts on each ray, by brute force: a list of
-    (prims, ts) arrays per ray. `phys` reuses a prepared FP32 PhysScene."""
+    (prims, ts) arrays per ray. `tmax` (a scalar or one per ray) bounds the
+    watertight hits; `phys` reuses a prepared FP32 PhysScene."""
     L = lib()
     verts = np.ascontiguousarray(verts, dtype=np.float32).reshape(-1)
     o = np.ascontiguousarray(o, dtype=np.float32).reshape(-1, 3)
@@ -296,12 +297,14 @@ def all_hits(verts, o, d, test="watertight", tmax=float("inf"), cap=64, phys=Non
         own = phys = PhysScene(verts, np.zeros(verts.size // 3, np.float32), None, "f32")
     prims = np.empty(cap, np.int32)
     ts = np.empty(cap, np.float32)
+    tm = np.broadcast_to(np.asarray(tmax, np.float32), (len(o),))
     out = []
     for i in range(len(o)):
         oi, di = np.ascontiguousarray(o[i]), np.ascontiguousarray(d[i])
         if test == "watertight":
-            k = L.or_phys_all_hits_f32(phys.h, oi, di, float(tmax), prims, ts, cap)
+            k = L.or_phys_all_hits_f32(phys.h, oi, di, float(tm[i]), prims, ts, cap)
         else:
+            # the reference's test has no tmax: hits are filtered by the caller
             k = L.or_compat_all_hits(verts, verts.size // 9, oi, di, prims, ts, cap)
         assert k <= cap, "more hits than the buffer holds"
         out.append((prims[:k].copy(), ts[:k].copy()))

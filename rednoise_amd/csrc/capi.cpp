@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <functional>
 #include <memory>
@@ -33,10 +34,29 @@ void DevBuf::release() {
 	bytes = 0;
 }
 
+bool fault_injected(const char *what) {
+	const char *env = std::getenv("RN_FAULT_INJECT");
+	if (!env || !*env)
+		return false;
+	const size_t len = std::strlen(what);
+	for (const char *p = env; *p;) {
+		const char *q = std::strchr(p, ',');
+		const size_t n = q ? (size_t)(q - p) : std::strlen(p);
+		if (n == len && std::strncmp(p, what, len) == 0)
+			return true;
+		if (!q)
+			break;
+		p = q + 1;
+	}
+	return false;
+}
+
 void *DevBuf::reserve(size_t n) {
 	if (n == 0)
 		n = 16;
 	if (n > bytes) {
+		if (fault_injected("alloc"))
+			throw Error("injected allocation failure (RN_FAULT_INJECT=alloc): " + std::to_string(n) + " bytes");
 		release();
 		RN_HIP(hipMalloc(&ptr, n));
 		bytes = n;
@@ -237,8 +257,11 @@ struct rn_scene {
 		if (bvh_valid && bvh.width == width && bvh.builder == builder)
 			return;
 		build_bvh(bvh, d_verts.as<float>(), h.n, width, s, builder);
-		if (bvh.depth > kMaxBvhDepth)
-			throw Error("BVH deeper than the traversal stack supports");
+		// checked before any traversal kernel runs on this BVH: a deeper tree
+		// would overflow the traversal stack (LDS entries + the spill stack)
+		if (bvh.depth > kMaxBvhDepth || fault_injected("bvh_depth"))
+			throw Error("BVH deeper than the traversal stack supports (depth " + std::to_string(bvh.depth) +
+			            ", limit " + std::to_string(kMaxBvhDepth) + ")");
 		bvh_valid = true;
 	}
 

@@ -21,6 +21,16 @@ struct Error : std::runtime_error {
 };
 
 void hip_check(hipError_t e, const char *what, const char *file, int line);
+
+// Failure injection for tests of the error paths. RN_FAULT_INJECT holds a
+// comma-separated list, read at every check:
+//   alloc                 every device and pinned allocation fails
+//   bvh_depth             every built BVH counts as deeper than the
+//                         traversal stack (the guard before any traversal)
+//   wavefront_iterations  the wavefront's iteration limit is one launch group
+// Each failure is raised on the host, before a kernel launch or between
+// synchronised launches, and reported like a real one.
+bool fault_injected(const char *what);
 #define RN_HIP(x) ::rn::hip_check((x), #x, __FILE__, __LINE__)
 
 // Grow-only device allocation owned by whoever holds it.

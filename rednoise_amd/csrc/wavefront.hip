@@ -518,7 +518,8 @@ __global__ void k_resolve(float *out, size_t n, float scale) {
 }
 
 struct WaveWork {
-	DevBuf state, hit, shadow, ctl, scenes, host_flag;
+	DevBuf state, hit, shadow, ctl, scenes;
+	unsigned *pinned_flag = nullptr; // host-pinned copy of the done flag, kept between renders
 	std::vector<hipEvent_t> events; // pool for per-stage timing
 	hipStream_t capture_stream = nullptr;
 	hipEvent_t join[2] = {nullptr, nullptr};
@@ -768,12 +769,19 @@ void wavefront_render_on(const WaveItem *items, int count, const WaveConfig &cfg
 	k_init<<<1, 1, 0, s>>>(q.ctl, total);
 	k_advance<<<1, 1, 0, s>>>(q, 1);
 	RN_HIP(hipGetLastError());
-	// the done flag is read into pinned memory, or into the stack when the
-	// render must allocate nothing
+	// the done flag is read into pinned memory (allocated once), or into the
+	// stack when the render must allocate nothing
 	unsigned stack_flag[2] = {0u, 0u};
 	unsigned *flag = stack_flag;
-	if (!ext)
-		RN_HIP(hipHostMalloc(reinterpret_cast<void **>(&flag), sizeof(unsigned) * 2, hipHostMallocDefault));
+	if (!ext) {
+		if (!ww.pinned_flag) {
+			if (fault_injected("alloc"))
+				throw Error("injected allocation failure (RN_FAULT_INJECT=alloc): pinned flag");
+			RN_HIP(hipHostMalloc(reinterpret_cast<void **>(&ww.pinned_flag), sizeof(unsigned) * 2,
+			                     hipHostMallocDefault));
+		}
+		flag = ww.pinned_flag;
+	}
 	int per = cfg.iters_per_graph > 0 ? cfg.iters_per_graph : 8;
 	per += per & 1;
 	hipGraphExec_t exec = nullptr;
@@ -793,6 +801,8 @@ void wavefront_render_on(const WaveItem *items, int count, const WaveConfig &cfg
 		for (int i = 0; i < count; i++)
 			maxd = std::max(maxd, items[i].params.max_depth);
 		uint64_t limit = ((total + cap - 1) / cap + 1) * (uint64_t)(maxd + 2) + 64;
+		if (fault_injected("wavefront_iterations"))
+			limit = 0; // the check below fires after the first launch group
 		while (true) {
 			if (iters > limit)
 				throw Error("wavefront: iteration limit exceeded (queue accounting error)");
@@ -827,16 +837,12 @@ void wavefront_render_on(const WaveItem *items, int count, const WaveConfig &cfg
 			(void)hipGraphExecDestroy(exec);
 		if (graph)
 			(void)hipGraphDestroy(graph);
-		if (flag != stack_flag)
-			(void)hipHostFree(flag);
 		throw;
 	}
 	if (exec)
 		RN_HIP(hipGraphExecDestroy(exec));
 	if (graph)
 		RN_HIP(hipGraphDestroy(graph));
-	if (flag != stack_flag)
-		RN_HIP(hipHostFree(flag));
 	for (int i = 0; i < count; i++) {
 		size_t n = (size_t)items[i].params.width * items[i].params.height * 3;
 		k_resolve<<<(unsigned)std::min<size_t>(2048, (n + 255) / 256), 256, 0, s>>>(items[i].out, n,

@@ -15,6 +15,12 @@ Outputs (small .npz fixtures next to this script):
 With --whitted (only these files):
   whitted_<asset>.npz  rn_render(RN_RAYTRACED) RGBA8 images (160x120, 96x64,
                        cam_z 4) and loadOBJ's smoothed vertex normals
+With --overflow (only this file):
+  modeR_overflow.npz   rn_render(RN_PATHTRACED) of the Cornell asset at
+                       1024x432, 1 spp, cam_z 4: rows 184-247, columns
+                       320-703, box pixels on both sides of y*W+x = 219,553,
+                       past which the reference's int seed (y*W+x)*9781+1
+                       (src/Renderer.cpp:1139) overflows
 With --photon (only these files):
   photon_<asset>.npz   renderPhotonMapped (20000 photons; no final gather and
                        3 gather rays) and renderBidirectional (16 samples,
@@ -178,8 +184,31 @@ def main() -> None:
     np.savez_compressed(os.path.join(HERE, "noise.npz"), p=pts, fbm4=out)
 
 
+def overflow(ref) -> None:
+    # The asset keeps its pixel size (f = 300 px) centred in the image: at
+    # 1024x432 the box's middle rows hold pixel 219,553 (row 214, x 417),
+    # so the band kept here has box pixels on both sides of the overflow.
+    W, H, spp = 1024, 432, 1
+    y0, y1, x0, x1 = 184, 248, 320, 704
+    assert y0 * W + x0 < 219_554 < (y1 - 1) * W + x1
+    sc = ref.rn_scene_load_obj(os.path.join(ASSETS, "cornell-box.obj").encode(), 0.35)
+    buf = np.empty(W * H * 4, np.uint8)
+    assert ref.rn_render(sc, 3, W, H, C.c_float(4.0), spp, buf) == 1
+    ref.rn_scene_free(sc)
+    img = buf.reshape(H, W, 4)[y0:y1, x0:x1]
+    np.savez_compressed(os.path.join(HERE, "modeR_overflow.npz"), rgba=img, width=W, height=H, spp=spp,
+                        rows=np.array([y0, y1]), cols=np.array([x0, x1]), cam_z=np.float32(4.0),
+                        scale=np.float32(0.35))
+    print("overflow band", img.shape, "distinct colours", len(np.unique(img.reshape(-1, 4), axis=0)))
+
+
 if __name__ == "__main__":
-    if "--whitted" in sys.argv or "--photon" in sys.argv:
+    if "--overflow" in sys.argv:
+        r = O.ref()
+        if r is None:
+            raise SystemExit("build the reference first: make -C oracle ref")
+        overflow(r)
+    elif "--whitted" in sys.argv or "--photon" in sys.argv:
         r = O.ref()
         if r is None:
             raise SystemExit("build the reference first: make -C oracle ref")

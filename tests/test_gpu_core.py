@@ -259,8 +259,11 @@ def _surface_rays_vs_brute(rn, oracle, sd, n_surf, n_in, seed):
     sdir = (to / dist[:, None]).astype(np.float32)
     tmax = (dist * 0.999).astype(np.float32)
     occ = sc.trace_rays(o_surf, sdir, tmax=tmax, query=rn.QUERY_ANY)
-    cp, ct, _, _ = sc.trace_rays(o_surf, sdir)
-    assert np.array_equal(occ.astype(bool), (cp >= 0) & (ct < tmax))
+    # against brute force over every triangle (any accepted hit before tmax)
+    brute = oracle.all_hits(v, o_surf, sdir, "watertight", tmax=tmax, cap=1 << 12, phys=ps)
+    want = np.array([len(p) > 0 for p, _ in brute])
+    assert np.array_equal(occ.astype(bool), want), int((occ.astype(bool) != want).sum())
+    assert 0 < want.sum() < len(want)  # both outcomes occur
 
 
 def test_flat_scene_closest_and_any_vs_brute(rn, oracle):
@@ -320,8 +323,20 @@ def test_compat_random_rays_vs_reference_bvh(rn, asset):
     assert (~same & ~band).sum() == 0
     hit = same & (prim >= 0)
     assert np.array_equal(t[hit], g["t"][hit])
+    # any hit against BVH::occluded of the compiled reference: equal on every
+    # ray except where the occluding hit lies in the edge band or within
+    # 1e-6 t of maxd (none occur on these rays; the count is printed)
     occ = sc.trace_rays(g["o"], g["d"], tmax=g["maxd"], query=rn.QUERY_ANY, tri_test=rn.TRI_COMPAT)
-    assert (occ == g["occluded"]).mean() > 0.995
+    from oracle import oracle as O
+    ob = np.load(os.path.join(GOLDEN, f"obj_{asset}.npz"))
+    hits = O.all_hits(ob["verts"], g["o"], g["d"], "compat", cap=1 << 12)
+    in_band = np.zeros(len(occ), bool)
+    for i, (ps, ts) in enumerate(hits):
+        near_maxd = np.abs(ts - g["maxd"][i]) <= 1e-6 * np.abs(ts)
+        in_band[i] = bool(near_maxd.any())
+    diff = occ != g["occluded"]
+    print(f"{asset}: any-hit mismatches {int(diff.sum())} of {len(occ)} ({int((diff & in_band).sum())} in the band)")
+    assert (diff & ~in_band).sum() == 0
 
 
 def cornell_setup(rn, oracle, W, H, spp, depth=5, rr=3, eps=0.05, seed=0):

@@ -232,3 +232,22 @@ def test_converges_to_fp64_oracle(rn):
         assert (z > 4).mean() <= 0.01, (integ, (z > 4).mean(), z.max())
         rel = math.sqrt(((mean - ref) ** 2).sum() / (ref ** 2).sum())
         assert rel < 0.02, (integ, rel)
+
+
+def test_rn_render_pathtraced_past_the_seed_overflow(rn):
+    """rn_render(RN_PATHTRACED) at 1024x432, 1 spp, against the compiled
+    reference's image (tests/golden/modeR_overflow.npz): box pixels before
+    and after pixel 219,553, past which the reference's int seed
+    (y*W+x)*9781+1 (src/Renderer.cpp:1139) overflows; the GPU seeds with the
+    same product modulo 2^32. Same criterion as the 160x120 image above."""
+    g = np.load(os.path.join(GOLDEN, "modeR_overflow.npz"))
+    W, H = int(g["width"]), int(g["height"])
+    (y0, y1), (x0, x1) = g["rows"], g["cols"]
+    sc = rn.Scene.load_obj(os.path.join(ASSETS, "cornell-box.obj"), float(g["scale"]))
+    img = sc.render("pathtraced", W, H, float(g["cam_z"]), int(g["spp"])).to_numpy()
+    band = img[y0:y1, x0:x1].astype(np.int32)
+    d = np.abs(band - g["rgba"].astype(np.int32)).max(axis=2)
+    ys, xs = np.mgrid[y0:y1, x0:x1]
+    past = ys * W + xs > 219_553
+    print("identical before / past the overflow:", (d[~past] == 0).mean(), (d[past] == 0).mean(), "max", d.max())
+    assert (d[past] == 0).mean() > 0.999 and (d[~past] == 0).mean() > 0.999 and (d <= 2).all()

@@ -8,7 +8,7 @@ import os
 import numpy as np
 import pytest
 
-from conftest import GOLDEN
+from conftest import ASSETS, GOLDEN
 
 # ----------------------------------------------------------------- Philox --
 
@@ -115,6 +115,20 @@ def test_compat_bvh_vs_reference_goldens(oracle, asset):
     assert np.array_equal(bp, g["brute_prim"])
     hb = bp >= 0
     assert np.array_equal(bt[hb], g["brute_t"][hb])
+    # BVH::occluded: the restated traversal and brute force over every
+    # triangle both give the reference's answer on every ray
+    import ctypes as C
+    L = oracle.lib()
+    v = np.ascontiguousarray(ob["verts"], np.float32)
+    h = L.or_compat_bvh_build(v, v.size // 9)
+    o3, d3 = g["o"].reshape(-1, 3), g["d"].reshape(-1, 3)
+    occ = np.array([L.or_compat_bvh_occluded(h, np.ascontiguousarray(o3[i]), np.ascontiguousarray(d3[i]),
+                                             float(g["maxd"][i]), -1) for i in range(len(o3))])
+    L.or_compat_bvh_free(h)
+    assert np.array_equal(occ, g["occluded"])
+    brute = np.array([int((ts < g["maxd"][i]).any()) for i, (_, ts) in
+                      enumerate(oracle.all_hits(v, o3, d3, "compat", cap=1 << 12))])
+    assert np.array_equal(brute, g["occluded"])
 
 
 def test_compat_primary_vs_reference_goldens(oracle):
@@ -432,3 +446,30 @@ def test_adjoint_is_linear_in_the_pixel_weights(oracle):
     assert np.allclose(ge, e1 + e2, rtol=1e-12, atol=1e-15)
     z, zz = ps.adjoint(cam, p, np.zeros_like(wgt))
     assert not z.any() and not zz.any()
+
+
+def test_mt19937_seeds_past_the_int_overflow(oracle):
+    """renderPathTraced seeds each pixel's std::mt19937 with the int
+    (y*W+x)*9781+1 (src/Renderer.cpp:1139), which overflows past pixel
+    219,553. The compiled reference's image at 1024x432, 1 spp holds box
+    pixels on both sides of that point (tests/golden/modeR_overflow.npz);
+    the restatement, which seeds with the same product taken modulo 2^32,
+    gives it pixel for pixel on both sides."""
+    import ctypes as C
+    g = np.load(os.path.join(GOLDEN, "modeR_overflow.npz"))
+    W, H = int(g["width"]), int(g["height"])
+    (y0, y1), (x0, x1) = g["rows"], g["cols"]
+    ob = np.load(os.path.join(GOLDEN, "obj_cornell.npz"))
+    from rednoise_amd import Scene
+    sc = Scene.load_obj(os.path.join(ASSETS, "cornell-box.obj"), float(g["scale"]))
+    assert np.array_equal(sc.triangles()[0].reshape(-1), ob["verts"])
+    cs = oracle.CompatArrays.from_scene(sc)
+    out = np.zeros(W * H * 4, np.uint8)
+    oracle.lib().or_compat_render_mt(C.byref(cs.s), W, H, float(g["cam_z"]), int(g["spp"]), int(y0), int(y1), out)
+    band = out.reshape(H, W, 4)[y0:y1, x0:x1]
+    ys, xs = np.mgrid[y0:y1, x0:x1]
+    past = ys * W + xs > 219_553
+    same = (band == g["rgba"]).all(axis=2)
+    print("identical before / past the overflow:", same[~past].mean(), same[past].mean())
+    assert past.sum() > 10_000 and (~past).sum() > 10_000
+    assert same.all()
