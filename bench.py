@@ -182,9 +182,12 @@ def adjoint_bench(rn, lib, torch, n_fd=64):
 
 def batch_bench(rn, torch):
     """Config 2: 64 seeded displaced spheres (1k-500k triangles, 256^2..1024^2,
-    4/16/64 spp, depth 8) in one set of persistent wavefront launches replayed
-    from a hipGraph, options picked by the autotuner; the plain megakernel
-    renders the same scenes one after another beside it."""
+    4/16/64 spp, depth 8) in one rn_render_batch call (one set of persistent
+    wavefront launches replayed from a cached hipGraph), options picked by the
+    autotuner. Beside it, the same 64 scenes as one wavefront call each (the
+    gain of batching alone) and as one plain-megakernel call each. Every call
+    downloads its images to the host; `*_gpu_s` is the GPU time of the
+    renders alone (events around the launches)."""
     from rednoise_amd import autotune, scenes
     t0 = time.perf_counter()
     specs = scenes.config2_batch(64, seed=2)
@@ -195,15 +198,32 @@ def batch_bench(rn, torch):
                       rn.RenderParams.make(s.width, s.height, s.spp, s.max_depth, 3, seed=0, ray_eps=0.03)))
     gen_s = time.perf_counter() - t0
     opts, report = autotune.autotune_batch(items, use_cache=False)
+    rn.release_workspace()  # the first call below captures its graph afresh
+    t0 = time.perf_counter()
     rn.render_batch(items, opts)
+    dt_first = time.perf_counter() - t0
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     imgs = rn.render_batch(items, opts)
     dt = time.perf_counter() - t0
     st = rn.last_stats()
     rays = st["rays_closest"] + st["rays_shadow"]
+    # the same scenes, one wavefront call each (same options)
+    for sc, cam, p in items:
+        p.integrator = rn.INTEGRATOR_WAVEFRONT
+        p.queue_paths, p.block_size, p.bvh_width = opts.queue_paths, opts.block_size, opts.bvh_width
+    for sc, cam, p in items[:2]:
+        sc.render_hdr(cam, p)
+    t0 = time.perf_counter()
+    each_rays, each_gpu = 0, 0.0
+    for sc, cam, p in items:
+        sc.render_hdr(cam, p)
+        s2 = rn.last_stats()
+        each_rays += s2["rays_closest"] + s2["rays_shadow"]
+        each_gpu += s2["render_ms"] * 1e-3
+    dt_each = time.perf_counter() - t0
     for sc, cam, p in items:  # the per-scene megakernel path runs on width-8 BVHs, built here
-        p.bvh_width = 8
+        p.integrator, p.queue_paths, p.block_size, p.bvh_width = rn.INTEGRATOR_MEGAKERNEL, 0, 0, 8
         sc.build(8)
     for sc, cam, p in items[:2]:
         sc.render_hdr(cam, p)
@@ -234,12 +254,19 @@ def batch_bench(rn, torch):
                           f"{int(f['block'])}x{int(f['block'])} block of each",
                 "batched_rel_rmse": round(subset_rmse(lambda k: imgs[k]), 5),
                 "plain_rel_rmse": round(subset_rmse(lambda k: plain_imgs[k]), 5)}
+    assert each_rays == rays == plain_rays, (rays, each_rays, plain_rays)
     return {"workload": f"config2: 64 scenes, {sum(sc.triangle_count for sc, _, _ in items):,} triangles, "
                         f"{sum(p.width * p.height * p.spp for _, _, p in items) / 1e6:.1f}M paths, depth 8, "
-                        "one persistent wavefront, hipGraph replay",
+                        "one rn_render_batch call: persistent wavefront launches replayed from a cached hipGraph",
             "batched_mrays_s": round(rays / dt / 1e6, 2), "batched_s": round(dt, 3),
+            "batched_gpu_s": round(st["render_ms"] * 1e-3, 3), "batched_first_call_s": round(dt_first, 3),
+            "iterations": st["iterations"],
+            "wavefront_one_call_each_mrays_s": round(each_rays / dt_each / 1e6, 2),
+            "wavefront_one_call_each_s": round(dt_each, 3), "wavefront_one_call_each_gpu_s": round(each_gpu, 3),
+            "batched_over_wavefront_one_call_each": round(dt_each / dt, 3),
+            "batched_over_wavefront_one_call_each_gpu": round(each_gpu / (st["render_ms"] * 1e-3), 3),
             "plain_sequential_mrays_s": round(plain_rays / dt_plain / 1e6, 2), "plain_s": round(dt_plain, 3),
-            "batched_over_plain": round((rays / dt) / (plain_rays / dt_plain), 3),
+            "batched_over_plain": round(dt_plain / dt, 3),
             "autotune": {"best": report["best"], "candidates": len(report["candidates"]),
                          "trial_spp_fraction": report["trial_spp_fraction"]},
             "scene_generation_s": round(gen_s, 1), "equal_rmse": rmse}
@@ -248,9 +275,9 @@ def batch_bench(rn, torch):
 def terrain_bench(rn, lib, torch):
     """Config 4: the 9,990,450-triangle terrain with 8 quad lights at
     3840x2160, 256 spp, depth 12, RR from depth 3: one step = GPU BVH build +
-    wavefront render (1 warm-up step, 1 timed). Then the terrain at 64 spp
-    batched with 16 config-2 scenes in one rn_render_batch call, beside the
-    same 17 renders made one call each."""
+    wavefront render (1 warm-up step, 1 timed). Then the terrain frame at the
+    same 256 spp batched with 16 config-2 scenes in one rn_render_batch call,
+    beside the same 17 renders made one wavefront call each."""
     from rednoise_amd import scenes
     t0 = time.perf_counter()
     sd = scenes.terrain_scene()
@@ -274,7 +301,7 @@ def terrain_bench(rn, lib, torch):
                              "wavefront_rel_rmse": round(rel_rmse(out.cpu().numpy(), f), 5)}
     del out
     specs = scenes.config2_batch(16, seed=2)
-    pt = rn.RenderParams.make(W, H, 64, depth, rr, seed=0, ray_eps=eps)
+    pt = rn.RenderParams.make(W, H, spp, depth, rr, seed=0, ray_eps=eps)
     items = [(sc, cam, pt)]
     for s in specs:
         sci = rn.Scene.from_arrays(s.scene.verts, s.scene.albedo, s.scene.emission)
@@ -289,17 +316,22 @@ def terrain_bench(rn, lib, torch):
     stb = rn.last_stats()
     rays_b = stb["rays_closest"] + stb["rays_shadow"]
     t0 = time.perf_counter()
-    rays_s = 0
+    rays_s, gpu_s = 0, 0.0
     for sci, ci, pi in items:
         pi.integrator = rn.INTEGRATOR_WAVEFRONT
         sci.render_hdr(ci, pi)
         s2 = rn.last_stats()
         rays_s += s2["rays_closest"] + s2["rays_shadow"]
+        gpu_s += s2["render_ms"] * 1e-3
     dts = time.perf_counter() - t0
     res["batch_with_16_config2"] = {
-        "workload": "terrain at 64 spp + 16 config-2 scenes, one rn_render_batch call (hipGraph replay)",
+        "workload": f"terrain 3840x2160 at {spp} spp + 16 config-2 scenes, one rn_render_batch call (cached "
+                    "hipGraph replay), against the same 17 renders as one wavefront call each",
         "batched_mrays_s": round(rays_b / dtb / 1e6, 2), "batched_s": round(dtb, 3),
-        "one_call_each_mrays_s": round(rays_s / dts / 1e6, 2), "one_call_each_s": round(dts, 3)}
+        "batched_gpu_s": round(stb["render_ms"] * 1e-3, 3),
+        "wavefront_one_call_each_mrays_s": round(rays_s / dts / 1e6, 2), "wavefront_one_call_each_s": round(dts, 3),
+        "wavefront_one_call_each_gpu_s": round(gpu_s, 3),
+        "batched_over_wavefront_one_call_each": round(dts / dtb, 3)}
     for sci, _, _ in items:
         sci.close()
     return res

@@ -24,6 +24,7 @@
 
 #include <algorithm>
 #include <cstddef>
+#include <cstring>
 #include <vector>
 
 namespace rn {
@@ -50,8 +51,8 @@ struct WCtl {
 	unsigned cur_count, gen_count;
 	unsigned next_count, sh_count; // one 64-bit word: k_shade bumps both with one atomic
 	unsigned long long gen_base, work_next, work_total;
-	unsigned done, iter;
-	unsigned pad[20];
+	unsigned done, iter, done_iter; // done_iter: value of iter when done was first set
+	unsigned pad[19];
 	unsigned trace_ctr[kStripes * kStripePad];
 	unsigned shadow_ctr[kStripes * kStripePad];
 };
@@ -96,6 +97,7 @@ __global__ void k_init(WCtl *ctl, unsigned long long total) {
 	ctl->work_total = total;
 	ctl->done = 0;
 	ctl->iter = 0;
+	ctl->done_iter = 0;
 }
 
 // Rotate queues and plan the next generation.
@@ -112,8 +114,13 @@ __global__ void k_advance(WQ q, int first) {
 	c->gen_base = c->work_next;
 	c->gen_count = gen;
 	c->work_next += gen;
+	// Once done, every later launch of the iteration finds empty queues and
+	// does nothing, so the host may enqueue more iterations than are needed.
+	const unsigned was_done = c->done;
 	c->done = (c->cur_count == 0 && gen == 0) ? 1u : 0u;
 	c->iter++;
+	if (!was_done && c->done)
+		c->done_iter = c->iter;
 }
 
 RN_DEV int find_scene(const WScene *sc, int n, unsigned long long g) {
@@ -519,7 +526,15 @@ __global__ void k_resolve(float *out, size_t n, float scale) {
 
 struct WaveWork {
 	DevBuf state, hit, shadow, ctl, scenes;
-	unsigned *pinned_flag = nullptr; // host-pinned copy of the done flag, kept between renders
+	unsigned *pinned_flag = nullptr; // host-pinned done, iter, done_iter; kept between renders
+	// The instantiated graph of the last graph render, with the launch
+	// arguments it has baked in; reused while they stay the same.
+	hipGraphExec_t exec = nullptr;
+	std::vector<uint64_t> exec_key;
+	// Iterations the last render needed, with its launch arguments: a render
+	// with the same ones enqueues that many before it first reads the flag.
+	std::vector<uint64_t> pred_key;
+	uint64_t pred_iters = 0;
 	std::vector<hipEvent_t> events; // pool for per-stage timing
 	hipStream_t capture_stream = nullptr;
 	hipEvent_t join[2] = {nullptr, nullptr};
@@ -543,6 +558,11 @@ WaveWork &work() {
 
 void wavefront_release() {
 	WaveWork &ww = work();
+	if (ww.exec)
+		(void)hipGraphExecDestroy(ww.exec); // it refers to the queues freed below
+	ww.exec = nullptr;
+	ww.exec_key.clear();
+	ww.pred_key.clear();
 	ww.state.release();
 	ww.hit.release();
 	ww.shadow.release();
@@ -664,6 +684,7 @@ void wavefront_render_on(const WaveItem *items, int count, const WaveConfig &cfg
 	}
 	const unsigned cap = (unsigned)std::max<unsigned long long>(std::min(total, want), 64ull);
 	WQ q;
+	std::memset(&q, 0, sizeof q); // padding too: the bytes of q key the cached graph
 	// carve the queues from the caller's memory, or reserve the library's own
 	char *carve = ext ? static_cast<char *>(ext->base) : nullptr;
 	auto take = [&](DevBuf &own, size_t bytes) -> void * {
@@ -769,43 +790,74 @@ void wavefront_render_on(const WaveItem *items, int count, const WaveConfig &cfg
 	k_init<<<1, 1, 0, s>>>(q.ctl, total);
 	k_advance<<<1, 1, 0, s>>>(q, 1);
 	RN_HIP(hipGetLastError());
-	// the done flag is read into pinned memory (allocated once), or into the
-	// stack when the render must allocate nothing
-	unsigned stack_flag[2] = {0u, 0u};
+	// done, iter and done_iter are read into pinned memory (allocated once),
+	// or into the stack when the render must allocate nothing
+	unsigned stack_flag[4] = {0u, 0u, 0u, 0u};
 	unsigned *flag = stack_flag;
 	if (!ext) {
 		if (!ww.pinned_flag) {
 			if (fault_injected("alloc"))
 				throw Error("injected allocation failure (RN_FAULT_INJECT=alloc): pinned flag");
-			RN_HIP(hipHostMalloc(reinterpret_cast<void **>(&ww.pinned_flag), sizeof(unsigned) * 2,
+			RN_HIP(hipHostMalloc(reinterpret_cast<void **>(&ww.pinned_flag), sizeof(unsigned) * 4,
 			                     hipHostMallocDefault));
 		}
 		flag = ww.pinned_flag;
 	}
 	int per = cfg.iters_per_graph > 0 ? cfg.iters_per_graph : 8;
 	per += per & 1;
+	// Everything an iteration's launches bake in: the queue descriptor (all
+	// pointers and sizes), the launch shapes and the kernel instances.
+	std::vector<uint64_t> key;
+	{
+		const unsigned char *qb = reinterpret_cast<const unsigned char *>(&q);
+		for (size_t off = 0; off < sizeof(WQ); off += 8) {
+			uint64_t w = 0;
+			std::memcpy(&w, qb + off, std::min<size_t>(8, sizeof(WQ) - off));
+			key.push_back(w);
+		}
+		key.push_back(((uint64_t)grid << 32) | sgrid);
+		key.push_back(((uint64_t)shgrid << 32) | (uint32_t)bs);
+		key.push_back(((uint64_t)deep << 2) | ((uint64_t)(count == 1) << 1) | (uint64_t)cfg.use_graph);
+		key.push_back((uint64_t)per);
+	}
 	hipGraphExec_t exec = nullptr;
-	hipGraph_t graph = nullptr;
-	try {
-		if (cfg.use_graph) {
+	if (cfg.use_graph) {
+		if (ww.exec && ww.exec_key == key) {
+			exec = ww.exec;
+		} else {
+			if (ww.exec)
+				RN_HIP(hipGraphExecDestroy(ww.exec));
+			ww.exec = nullptr;
+			ww.exec_key.clear();
+			hipGraph_t graph = nullptr;
 			RN_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
 			for (int it = 0; it < per; it++)
 				iteration(it & 1);
 			RN_HIP(hipStreamEndCapture(s, &graph));
-			RN_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+			hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+			(void)hipGraphDestroy(graph);
+			RN_HIP(e);
+			ww.exec = exec;
+			ww.exec_key = key;
 		}
-		uint64_t iters = 0;
-		// Every iteration retires or advances each queued path, so the loop
-		// ends within (batches of queue refills) x (max depth + 2) iterations.
-		int maxd = 1;
-		for (int i = 0; i < count; i++)
-			maxd = std::max(maxd, items[i].params.max_depth);
-		uint64_t limit = ((total + cap - 1) / cap + 1) * (uint64_t)(maxd + 2) + 64;
-		if (fault_injected("wavefront_iterations"))
-			limit = 0; // the check below fires after the first launch group
-		while (true) {
-			if (iters > limit)
-				throw Error("wavefront: iteration limit exceeded (queue accounting error)");
+	}
+	uint64_t iters = 0;
+	// Every iteration retires or advances each queued path, so the loop
+	// ends within (batches of queue refills) x (max depth + 2) iterations.
+	int maxd = 1;
+	for (int i = 0; i < count; i++)
+		maxd = std::max(maxd, items[i].params.max_depth);
+	uint64_t limit = ((total + cap - 1) / cap + 1) * (uint64_t)(maxd + 2) + 64;
+	if (fault_injected("wavefront_iterations"))
+		limit = 0; // the check below fires after the first launch group
+	// groups of `per` iterations enqueued before the flag is first read
+	uint64_t groups = 1;
+	if (ww.pred_key == key && ww.pred_iters > 0)
+		groups = (ww.pred_iters + (uint64_t)per - 1) / (uint64_t)per;
+	while (true) {
+		if (iters > limit)
+			throw Error("wavefront: iteration limit exceeded (queue accounting error)");
+		for (uint64_t g = 0; g < groups; g++) {
 			if (cfg.use_graph) {
 				RN_HIP(hipGraphLaunch(exec, s));
 			} else {
@@ -813,36 +865,31 @@ void wavefront_render_on(const WaveItem *items, int count, const WaveConfig &cfg
 					iteration(it & 1);
 			}
 			iters += (uint64_t)per;
-			RN_HIP(hipGetLastError());
-			RN_HIP(hipMemcpyAsync(flag, &q.ctl->done, sizeof(unsigned), hipMemcpyDeviceToHost, s));
-			RN_HIP(hipStreamSynchronize(s));
-			if (*flag)
-				break;
 		}
-		st.iterations = iters;
-		st.extend_launches = iters;
-		if (timed) {
-			double acc[3] = {0, 0, 0};
-			for (size_t k = 0; k < stage_of.size(); k++) {
-				float ms = 0;
-				RN_HIP(hipEventElapsedTime(&ms, ww.events[2 * k], ww.events[2 * k + 1]));
-				acc[stage_of[k]] += ms;
-			}
-			st.extend_ms = acc[0];
-			st.shade_ms = acc[1];
-			st.shadow_ms = acc[2];
-		}
-	} catch (...) {
-		if (exec)
-			(void)hipGraphExecDestroy(exec);
-		if (graph)
-			(void)hipGraphDestroy(graph);
-		throw;
+		groups = 1;
+		RN_HIP(hipGetLastError());
+		RN_HIP(hipMemcpyAsync(flag, &q.ctl->done, 3 * sizeof(unsigned), hipMemcpyDeviceToHost, s));
+		RN_HIP(hipStreamSynchronize(s));
+		if (flag[0])
+			break;
 	}
-	if (exec)
-		RN_HIP(hipGraphExecDestroy(exec));
-	if (graph)
-		RN_HIP(hipGraphDestroy(graph));
+	// iterations the render needed (the first advance only plans)
+	const uint64_t needed = flag[2] > 0 ? (uint64_t)flag[2] - 1 : iters;
+	ww.pred_key = key;
+	ww.pred_iters = needed;
+	st.iterations = needed;
+	st.extend_launches = needed;
+	if (timed) {
+		double acc[3] = {0, 0, 0};
+		for (size_t k = 0; k < stage_of.size(); k++) {
+			float ms = 0;
+			RN_HIP(hipEventElapsedTime(&ms, ww.events[2 * k], ww.events[2 * k + 1]));
+			acc[stage_of[k]] += ms;
+		}
+		st.extend_ms = acc[0];
+		st.shade_ms = acc[1];
+		st.shadow_ms = acc[2];
+	}
 	for (int i = 0; i < count; i++) {
 		size_t n = (size_t)items[i].params.width * items[i].params.height * 3;
 		k_resolve<<<(unsigned)std::min<size_t>(2048, (n + 255) / 256), 256, 0, s>>>(items[i].out, n,

@@ -96,10 +96,24 @@ def test_graph_replays_match_eager(rn):
     eager = sc.render_hdr(cam, base)
     g = rn.RenderParams.make(80, 60, 16, 6, 3, seed=4, ray_eps=0.05, integrator=rn.INTEGRATOR_WAVEFRONT,
                              queue_paths=8192, flags=rn.FLAG_USE_GRAPH)
-    for _ in range(3):
+    iters = set()
+    for _ in range(3):  # the first call captures the graph, the next replay the cached one
         rep = sc.render_hdr(cam, g)
         assert np.allclose(rep, eager, rtol=1e-5, atol=1e-6)
-        assert rn.last_stats()["iterations"] >= 8
+        iters.add(rn.last_stats()["iterations"])
+    assert len(iters) == 1 and min(iters) >= 8  # iterations the render needed, not launched
+    # another shape (new launch arguments: a graph of its own), then back
+    g2 = rn.RenderParams.make(64, 40, 8, 6, 3, seed=5, ray_eps=0.05, integrator=rn.INTEGRATOR_WAVEFRONT,
+                              queue_paths=4096, flags=rn.FLAG_USE_GRAPH)
+    e2 = rn.RenderParams.make(64, 40, 8, 6, 3, seed=5, ray_eps=0.05, integrator=rn.INTEGRATOR_WAVEFRONT,
+                              queue_paths=4096)
+    assert np.allclose(sc.render_hdr(cam, g2), sc.render_hdr(cam, e2), rtol=1e-5, atol=1e-6)
+    assert np.allclose(sc.render_hdr(cam, g), eager, rtol=1e-5, atol=1e-6)
+    # a different scene with the same shapes: the scene table is uploaded per call
+    from rednoise_amd import scenes as S
+    other = S.sphere_scene_tris(2000, seed=8)
+    so = rn.Scene.from_arrays(other.verts, other.albedo, other.emission)
+    assert np.allclose(so.render_hdr(cam, g), so.render_hdr(cam, base), rtol=1e-5, atol=1e-6)
 
 
 def test_bvh_width_4_matches_8(rn):
