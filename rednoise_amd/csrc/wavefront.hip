@@ -45,6 +45,7 @@ struct WScene {
 // Chunk counters of the trace kernels are striped over kStripes addresses,
 // one 128-byte line each, so waves contend on eight counters instead of one.
 constexpr int kStripes = 8;
+constexpr int kRing = 64; // >= the largest max_depth a render accepts
 constexpr int kStripePad = 32; // unsigneds per counter line
 
 struct WCtl {
@@ -55,6 +56,11 @@ struct WCtl {
 	unsigned pad[19];
 	unsigned trace_ctr[kStripes * kStripePad];
 	unsigned shadow_ctr[kStripes * kStripePad];
+	// Scene-index range [ring_lo, ring_hi] of the paths generated in each of
+	// the last kRing iterations (ring_hi < 0: none), and the one scene every
+	// path alive in the current iteration belongs to (-1: several or none).
+	int single;
+	int ring_lo[kRing], ring_hi[kRing];
 };
 static_assert(offsetof(WCtl, next_count) % 8 == 0 && offsetof(WCtl, sh_count) == offsetof(WCtl, next_count) + 4,
               "next_count and sh_count must form one aligned 64-bit word");
@@ -75,11 +81,7 @@ struct WQ {
 	WCtl *ctl;
 	const WScene *scenes;
 	int nscenes;
-	// a one-scene render passes its BVH, triangles and frame buffer here so
-	// the trace kernels hold them in scalar registers
-	const Bvh8Node *nodes0;
-	const float4 *tris0;
-	float *out0;
+	int max_depth; // the largest of the scenes' max_depth (paths live at most this many iterations)
 	unsigned capacity;
 	DevCounters *cnt;
 };
@@ -90,6 +92,9 @@ __global__ void k_fill(uint32_t *p, size_t n) {
 }
 
 __global__ void k_init(WCtl *ctl, unsigned long long total) {
+	for (int k = 0; k < kRing; k++)
+		ctl->ring_hi[k] = -1;
+	ctl->single = -1;
 	ctl->cur_count = ctl->gen_count = ctl->next_count = ctl->sh_count = 0;
 	reset_stripes(ctl);
 	ctl->gen_base = 0;
@@ -98,6 +103,18 @@ __global__ void k_init(WCtl *ctl, unsigned long long total) {
 	ctl->done = 0;
 	ctl->iter = 0;
 	ctl->done_iter = 0;
+}
+
+RN_DEV int find_scene(const WScene *sc, int n, unsigned long long g) {
+	int lo = 0, hi = n - 1;
+	while (lo < hi) {
+		int mid = (lo + hi + 1) >> 1;
+		if (sc[mid].first <= g)
+			lo = mid;
+		else
+			hi = mid - 1;
+	}
+	return lo;
 }
 
 // Rotate queues and plan the next generation.
@@ -121,19 +138,22 @@ __global__ void k_advance(WQ q, int first) {
 	c->iter++;
 	if (!was_done && c->done)
 		c->done_iter = c->iter;
+	// A path generated in iteration j is traced in iterations j .. j +
+	// max_depth - 1 at most, so the scenes of the paths alive in the coming
+	// iteration are among those generated in the last max_depth iterations,
+	// this one included.
+	const int slot = (int)(c->iter % (unsigned)q.max_depth);
+	c->ring_lo[slot] = gen > 0 ? find_scene(q.scenes, q.nscenes, c->gen_base) : 0;
+	c->ring_hi[slot] = gen > 0 ? find_scene(q.scenes, q.nscenes, c->gen_base + gen - 1) : -1;
+	int lo = 0x7FFFFFFF, hi = -1;
+	for (int k = 0; k < q.max_depth; k++)
+		if (c->ring_hi[k] >= 0) {
+			lo = min(lo, c->ring_lo[k]);
+			hi = max(hi, c->ring_hi[k]);
+		}
+	c->single = (hi >= 0 && lo == hi) ? lo : -1;
 }
 
-RN_DEV int find_scene(const WScene *sc, int n, unsigned long long g) {
-	int lo = 0, hi = n - 1;
-	while (lo < hi) {
-		int mid = (lo + hi + 1) >> 1;
-		if (sc[mid].first <= g)
-			lo = mid;
-		else
-			hi = mid - 1;
-	}
-	return lo;
-}
 
 // Queue records are written once and read back by the next stage, after
 // gigabytes of other traffic: non-temporal stores keep them from displacing
@@ -271,8 +291,12 @@ RN_DEV void trace_queue(const WQ &q, int cur, uint64_t *stk, const uint8_t *octa
 	unsigned next = 0, end = 0;
 	bool more = true, valid = false;
 	unsigned idx = 0;
-	const Bvh8Node *nodes = ONE ? q.nodes0 : nullptr;
-	const float4 *tris = ONE ? q.tris0 : nullptr;
+	// ONE: every queued ray belongs to scene c->single, whose BVH, triangles
+	// and frame buffer are held in scalar registers
+	const int one = ONE ? c->single : -1;
+	const Bvh8Node *nodes = ONE ? q.scenes[one].sc.nodes : nullptr;
+	const float4 *tris = ONE ? q.scenes[one].sc.tris : nullptr;
+	float *const out_one = ONE ? q.scenes[one].out : nullptr;
 	Trav<SPILL> t;
 	unsigned nrays = 0; // this lane's rays (at most the queue length, below 2^31)
 	while (true) {
@@ -343,7 +367,7 @@ RN_DEV void trace_queue(const WQ &q, int cur, uint64_t *stk, const uint8_t *octa
 					put4m(q.hitn + idx, make_float4(t.h.ng.x, t.h.ng.y, t.h.ng.z, t.h.t));
 			} else if (t.h.prim == 0) {
 				const float4 c4 = q.sc[idx];
-				float *out = ONE ? q.out0 : q.scenes[__float_as_uint(c4.w)].out;
+				float *out = ONE ? out_one : q.scenes[__float_as_uint(c4.w)].out;
 				float *fb = out + 3 * (size_t)__float_as_uint(q.so[idx].w);
 				atomicAdd(fb, c4.x);
 				atomicAdd(fb + 1, c4.y);
@@ -371,8 +395,18 @@ RN_DEV void trace_queue(const WQ &q, int cur, uint64_t *stk, const uint8_t *octa
 #define RN_OCT_TABLE 1
 #endif
 
+// Which instance runs an iteration: the one-scene instance when every queued
+// path belongs to one scene (always, for a one-scene render), the batch
+// instance otherwise; the other returns at once.
+template <bool ONE> RN_DEV bool instance_skips(const WQ &q) {
+	const int single = q.ctl->single;
+	return ONE ? single < 0 : single >= 0;
+}
+
 template <int LDS, int SPILL, int BS, bool ONE>
 __global__ __launch_bounds__(BS) RN_TRACE_ATTR void k_extend(WQ q, int cur) {
+	if (instance_skips<ONE>(q))
+		return;
 	__shared__ uint64_t stk[LDS * BS];
 	__shared__ uint8_t octab[RN_OCT_TABLE ? 2048 : 1];
 	if (RN_OCT_TABLE) {
@@ -384,6 +418,8 @@ __global__ __launch_bounds__(BS) RN_TRACE_ATTR void k_extend(WQ q, int cur) {
 
 template <int LDS, int SPILL, int BS, bool ONE>
 __global__ __launch_bounds__(BS) RN_TRACE_ATTR void k_shadow(WQ q) {
+	if (instance_skips<ONE>(q))
+		return;
 	__shared__ uint64_t stk[LDS * BS];
 	__shared__ uint8_t octab[RN_OCT_TABLE ? 2048 : 1];
 	if (RN_OCT_TABLE) {
@@ -406,6 +442,9 @@ constexpr int kShadeWaves = kShadeBlock / 64;
 
 template <bool ONE>
 __global__ __launch_bounds__(kShadeBlock) void k_shade(WQ q, int cur) {
+	if (instance_skips<ONE>(q))
+		return;
+	const unsigned one = ONE ? (unsigned)q.ctl->single : 0u;
 	__shared__ unsigned wcnt[2][kShadeWaves];
 	__shared__ unsigned bbase[2];
 	WCtl *c = q.ctl;
@@ -435,7 +474,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(WQ q, int cur) {
 				d4 = get4(D + i);
 				beta = xyz(b4);
 				// one scene: a uniform address, loaded once into scalar registers
-				const WScene &ws = q.scenes[ONE ? 0u : (meta & 0xFFFFu)];
+				const WScene &ws = q.scenes[ONE ? one : (meta & 0xFFFFu)];
 				const DevScene &sc = ws.sc;
 				const DevParams &p = ws.p;
 				float3 o = xyz(o4), d = xyz(d4);
@@ -715,9 +754,14 @@ void wavefront_render_on(const WaveItem *items, int count, const WaveConfig &cfg
 	RN_HIP(hipMemcpyAsync(dsc, hs.data(), hs.size() * sizeof(WScene), hipMemcpyHostToDevice, s));
 	q.scenes = dsc;
 	q.nscenes = count;
-	q.nodes0 = hs[0].sc.nodes;
-	q.tris0 = hs[0].sc.tris;
-	q.out0 = hs[0].out;
+	{
+		int md = 1;
+		for (int i = 0; i < count; i++)
+			md = std::max(md, items[i].params.max_depth);
+		if (md > kRing)
+			throw Error("wavefront: max_depth above " + std::to_string(kRing));
+		q.max_depth = md;
+	}
 	q.capacity = cap;
 	q.cnt = cnt;
 
@@ -765,17 +809,15 @@ void wavefront_render_on(const WaveItem *items, int count, const WaveConfig &cfg
 	} while (0)
 #define RN_LAUNCH(K, ...)                                                                                             \
 	do {                                                                                                               \
-		if (count == 1)                                                                                                \
-			RN_LAUNCH1(K, true, __VA_ARGS__);                                                                          \
-		else                                                                                                           \
+		RN_LAUNCH1(K, true, __VA_ARGS__);                                                                              \
+		if (count > 1)                                                                                                 \
 			RN_LAUNCH1(K, false, __VA_ARGS__);                                                                         \
 	} while (0)
 		RN_LAUNCH(k_extend, q, cur);
 		mark(0, false);
 		mark(1, true);
-		if (count == 1)
-			k_shade<true><<<shgrid, kShadeBlock, 0, s>>>(q, cur);
-		else
+		k_shade<true><<<shgrid, kShadeBlock, 0, s>>>(q, cur);
+		if (count > 1)
 			k_shade<false><<<shgrid, kShadeBlock, 0, s>>>(q, cur);
 		mark(1, false);
 		mark(2, true);
