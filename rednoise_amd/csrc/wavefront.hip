@@ -389,6 +389,22 @@ RN_DEV void trace_queue(const WQ &q, int cur, uint64_t *stk, const uint8_t *octa
 #define RN_TRACE_ATTR __attribute__((amdgpu_waves_per_eu((SPILL == 0 && ONE) ? RN_TRACE_WPE : 0, \
                                                          (SPILL == 0 && ONE) ? RN_TRACE_WPE : 0)))
 
+// LDS stack entries and occupancy target of the any-hit kernel, set apart
+// from the closest-hit kernel's. With 10 entries it fits 7 waves per SIMD
+// (72 VGPRs, no scratch, 7 blocks of 22.5 KB LDS), but at 1M triangles that
+// runs 1-2% slower than 12 entries at 6 waves (18.1-18.2 against 17.9-18.0
+// ms per frame), so the defaults are those of the closest-hit kernel.
+#ifndef RN_SHADOW_LDS
+#define RN_SHADOW_LDS 12
+#endif
+#ifndef RN_SHADOW_WPE
+#define RN_SHADOW_WPE 6
+#endif
+constexpr int kShadowLds = RN_SHADOW_LDS;
+constexpr int kShadowSpill = kMaxBvhDepth - kShadowLds - 1;
+#define RN_SHADOW_ATTR __attribute__((amdgpu_waves_per_eu((SPILL == 0 && ONE) ? RN_SHADOW_WPE : 0, \
+                                                          (SPILL == 0 && ONE) ? RN_SHADOW_WPE : 0)))
+
 // The octant permutation of a node's hit mask as a table in LDS (one byte
 // load per node visit instead of three conditional bit swaps).
 #ifndef RN_OCT_TABLE
@@ -417,7 +433,7 @@ __global__ __launch_bounds__(BS) RN_TRACE_ATTR void k_extend(WQ q, int cur) {
 }
 
 template <int LDS, int SPILL, int BS, bool ONE>
-__global__ __launch_bounds__(BS) RN_TRACE_ATTR void k_shadow(WQ q) {
+__global__ __launch_bounds__(BS) RN_SHADOW_ATTR void k_shadow(WQ q) {
 	if (instance_skips<ONE>(q))
 		return;
 	__shared__ uint64_t stk[LDS * BS];
@@ -768,7 +784,7 @@ void wavefront_render_on(const WaveItem *items, int count, const WaveConfig &cfg
 	int dev = 0, n_cu = 0;
 	RN_HIP(hipGetDevice(&dev));
 	RN_HIP(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
-	bool deep = max_depth - 1 > kLdsDepth;
+	const bool deep = max_depth - 1 > kLdsDepth, deep_sh = max_depth - 1 > kShadowLds;
 
 	// Stage timing: events bracket each launch; pairs are summed at the end.
 	const bool timed = cfg.time_stages && !cfg.use_graph;
@@ -796,24 +812,24 @@ void wavefront_render_on(const WaveItem *items, int count, const WaveConfig &cfg
 		k_generate<<<sgrid, 256, 0, s>>>(q, cur);
 		mark(1, false);
 		mark(0, true);
-#define RN_LAUNCH1(K, ONE, ...)                                                                                       \
+#define RN_LAUNCH1(K, LDSD, SPD, DEEP, ONE, ...)                                                                      \
 	do {                                                                                                               \
-		if (deep && bs == 128)                                                                                         \
-			K<kLdsDepth, kSpillDepth, 128, ONE><<<grid, 128, 0, s>>>(__VA_ARGS__);                                     \
-		else if (deep)                                                                                                 \
-			K<kLdsDepth, kSpillDepth, 256, ONE><<<grid, 256, 0, s>>>(__VA_ARGS__);                                     \
+		if (DEEP && bs == 128)                                                                                         \
+			K<LDSD, SPD, 128, ONE><<<grid, 128, 0, s>>>(__VA_ARGS__);                                                  \
+		else if (DEEP)                                                                                                 \
+			K<LDSD, SPD, 256, ONE><<<grid, 256, 0, s>>>(__VA_ARGS__);                                                  \
 		else if (bs == 128)                                                                                            \
-			K<kLdsDepth, 0, 128, ONE><<<grid, 128, 0, s>>>(__VA_ARGS__);                                               \
+			K<LDSD, 0, 128, ONE><<<grid, 128, 0, s>>>(__VA_ARGS__);                                                    \
 		else                                                                                                           \
-			K<kLdsDepth, 0, 256, ONE><<<grid, 256, 0, s>>>(__VA_ARGS__);                                               \
+			K<LDSD, 0, 256, ONE><<<grid, 256, 0, s>>>(__VA_ARGS__);                                                    \
 	} while (0)
-#define RN_LAUNCH(K, ...)                                                                                             \
+#define RN_LAUNCH(K, LDSD, SPD, DEEP, ...)                                                                            \
 	do {                                                                                                               \
-		RN_LAUNCH1(K, true, __VA_ARGS__);                                                                              \
+		RN_LAUNCH1(K, LDSD, SPD, DEEP, true, __VA_ARGS__);                                                             \
 		if (count > 1)                                                                                                 \
-			RN_LAUNCH1(K, false, __VA_ARGS__);                                                                         \
+			RN_LAUNCH1(K, LDSD, SPD, DEEP, false, __VA_ARGS__);                                                        \
 	} while (0)
-		RN_LAUNCH(k_extend, q, cur);
+		RN_LAUNCH(k_extend, kLdsDepth, kSpillDepth, deep, q, cur);
 		mark(0, false);
 		mark(1, true);
 		k_shade<true><<<shgrid, kShadeBlock, 0, s>>>(q, cur);
@@ -821,7 +837,7 @@ void wavefront_render_on(const WaveItem *items, int count, const WaveConfig &cfg
 			k_shade<false><<<shgrid, kShadeBlock, 0, s>>>(q, cur);
 		mark(1, false);
 		mark(2, true);
-		RN_LAUNCH(k_shadow, q);
+		RN_LAUNCH(k_shadow, kShadowLds, kShadowSpill, deep_sh, q);
 		mark(2, false);
 #undef RN_LAUNCH
 #undef RN_LAUNCH1
@@ -859,7 +875,8 @@ void wavefront_render_on(const WaveItem *items, int count, const WaveConfig &cfg
 		}
 		key.push_back(((uint64_t)grid << 32) | sgrid);
 		key.push_back(((uint64_t)shgrid << 32) | (uint32_t)bs);
-		key.push_back(((uint64_t)deep << 2) | ((uint64_t)(count == 1) << 1) | (uint64_t)cfg.use_graph);
+		key.push_back(((uint64_t)deep_sh << 3) | ((uint64_t)deep << 2) | ((uint64_t)(count == 1) << 1) |
+		              (uint64_t)cfg.use_graph);
 		key.push_back((uint64_t)per);
 	}
 	hipGraphExec_t exec = nullptr;
