@@ -2,21 +2,22 @@
 // "Megakernels Considered Harmful", HPG 2013), for one scene or a batch of
 // scenes of uneven size that share one path queue.
 //
-// Per iteration five kernels run:
+// Per iteration four kernels run:
 //   generate - appends freshly generated camera paths to the queue;
-//   extend   - closest hit of every queued path (SoA state);
+//   trace    - closest hit of every queued path (SoA state), then the any-hit
+//              rays the previous iteration's shade stage pushed (unoccluded
+//              contributions are added to the frame buffer with float
+//              atomics);
 //   shade    - emission, NEE light sample, bounce and Russian roulette;
 //              shadow rays and continuing paths are pushed with one wave64
-//              ballot and one atomic per wave (compaction keeps the next
-//              queue dense);
-//   shadow   - any-hit queries; unoccluded contributions are added to the
-//              frame buffer with float atomics;
+//              ballot per wave and one atomic per block (compaction keeps
+//              the next queue dense);
 //   advance  - one lane: rotates the queues and plans the next generation so
 //              the queue stays full until the work runs out.
-// The two trace kernels are persistent: a wave takes chunks of queue entries
-// from an atomic counter and gives a lane whose ray has finished the next
-// ray of the chunk as soon as enough lanes are idle, so a wave no longer
-// runs as long as its slowest ray once per 64 rays.
+// The trace kernel is persistent: a wave takes chunks of queue entries from
+// an atomic counter and gives a lane whose ray has finished the next ray of
+// the chunk as soon as enough lanes are idle, so a wave no longer runs as
+// long as its slowest ray once per 64 rays.
 // Queue sizes live in device memory, so the iteration can be captured once in
 // a hipGraph and replayed; the host only polls a done flag between replays.
 #include "rn_pathp.h"
@@ -45,7 +46,13 @@ struct WScene {
 // Chunk counters of the trace kernels are striped over kStripes addresses,
 // one 128-byte line each, so waves contend on eight counters instead of one.
 constexpr int kStripes = 8;
-constexpr int kRing = 64; // >= the largest max_depth a render accepts
+constexpr int kRing = 72; // > the largest max_depth a render accepts
+
+// The any-hit rays an iteration's shade stage pushes are traced in the next
+// iteration's trace launch, after its closest-hit rays (they are
+// independent: the shadow queue is rewritten only by the next shade stage).
+// One persistent launch per iteration instead of two has one drain tail
+// instead of two: 38.0 against 39.0 ms of tracing per 1M-triangle 1080p frame.
 constexpr int kStripePad = 32; // unsigneds per counter line
 
 struct WCtl {
@@ -53,7 +60,8 @@ struct WCtl {
 	unsigned next_count, sh_count; // one 64-bit word: k_shade bumps both with one atomic
 	unsigned long long gen_base, work_next, work_total;
 	unsigned done, iter, done_iter; // done_iter: value of iter when done was first set
-	unsigned pad[19];
+	unsigned sh_pending;            // any-hit rays to trace in the coming iteration
+	unsigned pad[18];
 	unsigned trace_ctr[kStripes * kStripePad];
 	unsigned shadow_ctr[kStripes * kStripePad];
 	// Scene-index range [ring_lo, ring_hi] of the paths generated in each of
@@ -103,6 +111,7 @@ __global__ void k_init(WCtl *ctl, unsigned long long total) {
 	ctl->done = 0;
 	ctl->iter = 0;
 	ctl->done_iter = 0;
+	ctl->sh_pending = 0;
 }
 
 RN_DEV int find_scene(const WScene *sc, int n, unsigned long long g) {
@@ -122,6 +131,7 @@ __global__ void k_advance(WQ q, int first) {
 	WCtl *c = q.ctl;
 	if (!first)
 		c->cur_count = c->next_count;
+	c->sh_pending = first ? 0u : c->sh_count;
 	c->next_count = 0;
 	c->sh_count = 0;
 	reset_stripes(c);
@@ -134,19 +144,21 @@ __global__ void k_advance(WQ q, int first) {
 	// Once done, every later launch of the iteration finds empty queues and
 	// does nothing, so the host may enqueue more iterations than are needed.
 	const unsigned was_done = c->done;
-	c->done = (c->cur_count == 0 && gen == 0) ? 1u : 0u;
+	c->done = (c->cur_count == 0 && gen == 0 && c->sh_pending == 0) ? 1u : 0u;
 	c->iter++;
 	if (!was_done && c->done)
 		c->done_iter = c->iter;
 	// A path generated in iteration j is traced in iterations j .. j +
-	// max_depth - 1 at most, so the scenes of the paths alive in the coming
-	// iteration are among those generated in the last max_depth iterations,
-	// this one included.
-	const int slot = (int)(c->iter % (unsigned)q.max_depth);
+	// max_depth - 1 at most (its last any-hit ray one iteration later when
+	// that ray shares the next trace launch), so the scenes of the rays of
+	// the coming iteration are among those generated in the last `window`
+	// iterations, this one included.
+	const int window = q.max_depth + 1;
+	const int slot = (int)(c->iter % (unsigned)window);
 	c->ring_lo[slot] = gen > 0 ? find_scene(q.scenes, q.nscenes, c->gen_base) : 0;
 	c->ring_hi[slot] = gen > 0 ? find_scene(q.scenes, q.nscenes, c->gen_base + gen - 1) : -1;
 	int lo = 0x7FFFFFFF, hi = -1;
-	for (int k = 0; k < q.max_depth; k++)
+	for (int k = 0; k < window; k++)
 		if (c->ring_hi[k] >= 0) {
 			lo = min(lo, c->ring_lo[k]);
 			hi = max(hi, c->ring_hi[k]);
@@ -276,14 +288,17 @@ static_assert(kRefillMin >= 1 && kRefillMin <= 64, "refill threshold is a lane c
 template <int QUERY, int LDS, int SPILL, int BS, bool ONE>
 RN_DEV void trace_queue(const WQ &q, int cur, uint64_t *stk, const uint8_t *octab) {
 	WCtl *c = q.ctl;
-	const unsigned n = QUERY == 0 ? c->cur_count + c->gen_count : c->sh_count;
+	const unsigned n = QUERY == 0 ? c->cur_count + c->gen_count : c->sh_pending;
 	unsigned *ctrs = QUERY == 0 ? c->trace_ctr : c->shadow_ctr;
 	// chunk j = stripe + kStripes * k is the k-th chunk of stripe j % kStripes;
 	// a wave starts on its own stripe and moves on when it is exhausted
 	const unsigned waves = gridDim.x * (unsigned)(BS / 64);
 	const unsigned chunk = min(kChunkMax, max(kChunkMin, n / (waves * kChunksPerWave)));
 	const unsigned nch = (n + chunk - 1) / chunk;
-	unsigned stripe = (blockIdx.x * (unsigned)(BS / 64) + (threadIdx.x >> 6)) % (unsigned)kStripes;
+	// wave-uniform values are taken with readfirstlane (every lane is active
+	// where they are read), so they live in scalar registers
+	unsigned stripe =
+	    __builtin_amdgcn_readfirstlane((blockIdx.x * (unsigned)(BS / 64) + (threadIdx.x >> 6)) % (unsigned)kStripes);
 	const float4 *O = QUERY == 0 ? q.o[cur] : q.so;
 	const float4 *D = QUERY == 0 ? q.d[cur] : q.sd;
 	const float4 *M = QUERY == 0 ? q.b[cur] : q.sc; // .w: meta (closest) or scene index (shadow)
@@ -320,8 +335,8 @@ RN_DEV void trace_queue(const WQ &q, int cur, uint64_t *stk, const uint8_t *octa
 						stripe = (stripe + 1u) % (unsigned)kStripes;
 					}
 				}
-				cb = __shfl(cb, 0);
-				stripe = __shfl(stripe, 0);
+				cb = __builtin_amdgcn_readfirstlane(cb);
+				stripe = __builtin_amdgcn_readfirstlane(stripe);
 				if (cb == 0xFFFFFFFFu) {
 					more = false;
 				} else {
@@ -389,22 +404,6 @@ RN_DEV void trace_queue(const WQ &q, int cur, uint64_t *stk, const uint8_t *octa
 #define RN_TRACE_ATTR __attribute__((amdgpu_waves_per_eu((SPILL == 0 && ONE) ? RN_TRACE_WPE : 0, \
                                                          (SPILL == 0 && ONE) ? RN_TRACE_WPE : 0)))
 
-// LDS stack entries and occupancy target of the any-hit kernel, set apart
-// from the closest-hit kernel's. With 10 entries it fits 7 waves per SIMD
-// (72 VGPRs, no scratch, 7 blocks of 22.5 KB LDS), but at 1M triangles that
-// runs 1-2% slower than 12 entries at 6 waves (18.1-18.2 against 17.9-18.0
-// ms per frame), so the defaults are those of the closest-hit kernel.
-#ifndef RN_SHADOW_LDS
-#define RN_SHADOW_LDS 12
-#endif
-#ifndef RN_SHADOW_WPE
-#define RN_SHADOW_WPE 6
-#endif
-constexpr int kShadowLds = RN_SHADOW_LDS;
-constexpr int kShadowSpill = kMaxBvhDepth - kShadowLds - 1;
-#define RN_SHADOW_ATTR __attribute__((amdgpu_waves_per_eu((SPILL == 0 && ONE) ? RN_SHADOW_WPE : 0, \
-                                                          (SPILL == 0 && ONE) ? RN_SHADOW_WPE : 0)))
-
 // The octant permutation of a node's hit mask as a table in LDS (one byte
 // load per node visit instead of three conditional bit swaps).
 #ifndef RN_OCT_TABLE
@@ -419,8 +418,10 @@ template <bool ONE> RN_DEV bool instance_skips(const WQ &q) {
 	return ONE ? single < 0 : single >= 0;
 }
 
+// Both queries in one persistent launch: a wave drains the closest-hit
+// queue, then the any-hit queue of the previous iteration.
 template <int LDS, int SPILL, int BS, bool ONE>
-__global__ __launch_bounds__(BS) RN_TRACE_ATTR void k_extend(WQ q, int cur) {
+__global__ __launch_bounds__(BS) RN_TRACE_ATTR void k_trace(WQ q, int cur) {
 	if (instance_skips<ONE>(q))
 		return;
 	__shared__ uint64_t stk[LDS * BS];
@@ -430,18 +431,6 @@ __global__ __launch_bounds__(BS) RN_TRACE_ATTR void k_extend(WQ q, int cur) {
 		__syncthreads();
 	}
 	trace_queue<0, LDS, SPILL, BS, ONE>(q, cur, stk, RN_OCT_TABLE ? octab : nullptr);
-}
-
-template <int LDS, int SPILL, int BS, bool ONE>
-__global__ __launch_bounds__(BS) RN_SHADOW_ATTR void k_shadow(WQ q) {
-	if (instance_skips<ONE>(q))
-		return;
-	__shared__ uint64_t stk[LDS * BS];
-	__shared__ uint8_t octab[RN_OCT_TABLE ? 2048 : 1];
-	if (RN_OCT_TABLE) {
-		fill_oct_table(octab, threadIdx.x, BS);
-		__syncthreads();
-	}
 	trace_queue<1, LDS, SPILL, BS, ONE>(q, 0, stk, RN_OCT_TABLE ? octab : nullptr);
 }
 
@@ -784,7 +773,7 @@ void wavefront_render_on(const WaveItem *items, int count, const WaveConfig &cfg
 	int dev = 0, n_cu = 0;
 	RN_HIP(hipGetDevice(&dev));
 	RN_HIP(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
-	const bool deep = max_depth - 1 > kLdsDepth, deep_sh = max_depth - 1 > kShadowLds;
+	const bool deep = max_depth - 1 > kLdsDepth;
 
 	// Stage timing: events bracket each launch; pairs are summed at the end.
 	const bool timed = cfg.time_stages && !cfg.use_graph;
@@ -829,16 +818,13 @@ void wavefront_render_on(const WaveItem *items, int count, const WaveConfig &cfg
 		if (count > 1)                                                                                                 \
 			RN_LAUNCH1(K, LDSD, SPD, DEEP, false, __VA_ARGS__);                                                        \
 	} while (0)
-		RN_LAUNCH(k_extend, kLdsDepth, kSpillDepth, deep, q, cur);
+		RN_LAUNCH(k_trace, kLdsDepth, kSpillDepth, deep, q, cur);
 		mark(0, false);
 		mark(1, true);
 		k_shade<true><<<shgrid, kShadeBlock, 0, s>>>(q, cur);
 		if (count > 1)
 			k_shade<false><<<shgrid, kShadeBlock, 0, s>>>(q, cur);
 		mark(1, false);
-		mark(2, true);
-		RN_LAUNCH(k_shadow, kShadowLds, kShadowSpill, deep_sh, q);
-		mark(2, false);
 #undef RN_LAUNCH
 #undef RN_LAUNCH1
 		k_advance<<<1, 1, 0, s>>>(q, 0);
@@ -875,8 +861,7 @@ void wavefront_render_on(const WaveItem *items, int count, const WaveConfig &cfg
 		}
 		key.push_back(((uint64_t)grid << 32) | sgrid);
 		key.push_back(((uint64_t)shgrid << 32) | (uint32_t)bs);
-		key.push_back(((uint64_t)deep_sh << 3) | ((uint64_t)deep << 2) | ((uint64_t)(count == 1) << 1) |
-		              (uint64_t)cfg.use_graph);
+		key.push_back(((uint64_t)deep << 2) | ((uint64_t)(count == 1) << 1) | (uint64_t)cfg.use_graph);
 		key.push_back((uint64_t)per);
 	}
 	hipGraphExec_t exec = nullptr;
