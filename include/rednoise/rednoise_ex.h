@@ -188,7 +188,7 @@ typedef struct {
 	uint64_t rays_shadow;   /* any-hit queries issued */
 	uint64_t paths;         /* camera paths traced */
 	double trace_kernel_ms; /* summed duration of the dominant trace kernel */
-	uint64_t trace_kernel_launches;
+	uint64_t trace_kernel_launches; /* wavefront: iterations enqueued, including any that found the queues empty */
 	int bvh_nodes;
 	int bvh_depth;
 	uint64_t nodes_closest, tris_closest; /* with RN_FLAG_COUNT_TRAVERSAL */

@@ -26,15 +26,15 @@ struct GradSink {
 };
 
 RN_DEV void grad_add(float *glob, float *lds, const int *hot_slot, int prim, float3 g) {
-	int hs = hot_slot[prim];
+	int hs = gload_i(hot_slot + prim);
 	if (hs >= 0) {
-		atomicAdd(&lds[3 * hs], g.x);
-		atomicAdd(&lds[3 * hs + 1], g.y);
-		atomicAdd(&lds[3 * hs + 2], g.z);
+		atomic_add_lds(&lds[3 * hs], g.x);
+		atomic_add_lds(&lds[3 * hs + 1], g.y);
+		atomic_add_lds(&lds[3 * hs + 2], g.z);
 	} else {
-		atomicAdd(&glob[3 * prim], g.x);
-		atomicAdd(&glob[3 * prim + 1], g.y);
-		atomicAdd(&glob[3 * prim + 2], g.z);
+		atomic_add_global(&glob[3 * prim], g.x);
+		atomic_add_global(&glob[3 * prim + 1], g.y);
+		atomic_add_global(&glob[3 * prim + 2], g.z);
 	}
 }
 
@@ -182,9 +182,9 @@ __global__ __launch_bounds__(kMBlock) void k_adjoint(DevScene sc, DevCamera cam,
 	for (int i = threadIdx.x; i < 3 * n_hot; i += kMBlock) {
 		int prim = hot_prim[i / 3], c = i % 3;
 		if (la[i] != 0.f)
-			atomicAdd(&g_albedo[3 * prim + c], la[i]);
+			atomic_add_global(&g_albedo[3 * prim + c], la[i]);
 		if (le[i] != 0.f)
-			atomicAdd(&g_emission[3 * prim + c], le[i]);
+			atomic_add_global(&g_emission[3 * prim + c], le[i]);
 	}
 }
 

@@ -326,6 +326,24 @@ RN_DEV float4 gload_f4(const float4 *base, uint32_t i) {
 	return make_float4(v.x, v.y, v.z, v.w);
 }
 
+RN_DEV float gload_f(const float *p) { return *(__attribute__((address_space(1))) const float *)p; }
+RN_DEV int gload_i(const int *p) { return *(__attribute__((address_space(1))) const int *)p; }
+
+// Float adds into global memory and into LDS through pointers the compiler
+// sees as generic (read from a scene table, or held in a struct). Through a
+// generic pointer an atomic is a flat instruction, which the hardware also
+// counts on the LDS counter: the next wait for an LDS access (the traversal
+// stack) then waits for the atomic's round trip to memory too. Naming the
+// address space makes them global and LDS atomics. Relaxed, like atomicAdd.
+RN_DEV void atomic_add_global(float *p, float v) {
+	__hip_atomic_fetch_add((__attribute__((address_space(1))) float *)p, v, __ATOMIC_RELAXED,
+	                       __HIP_MEMORY_SCOPE_AGENT);
+}
+RN_DEV void atomic_add_lds(float *p, float v) {
+	__hip_atomic_fetch_add((__attribute__((address_space(3))) float *)p, v, __ATOMIC_RELAXED,
+	                       __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
 RN_DEV void load_node_g(const Bvh8Node *nodes, uint32_t i, uint32_t w[20]) {
 	g_nu4 *p = (g_nu4 *)(nodes + i);
 #pragma unroll
@@ -489,7 +507,12 @@ RN_DEV bool trav_step(Trav<SPILL> &s, const Bvh8Node *__restrict__ nodes, const 
 			return true;
 		}
 		s.sp--;
-		s.grp = (SPILL == 0 || s.sp < LDS_DEPTH) ? lds[s.sp * ldsStride + lane] : s.spill[s.sp - LDS_DEPTH];
+		// the LDS entry is read through an LDS pointer: a select between the
+		// two addresses would otherwise become one flat load
+		if (SPILL == 0 || s.sp < LDS_DEPTH)
+			s.grp = *(__attribute__((address_space(3))) uint64_t *)(lds + s.sp * ldsStride + lane);
+		else
+			s.grp = s.spill[s.sp - LDS_DEPTH];
 	}
 	return false;
 }

@@ -39,7 +39,7 @@ RN_DEV int pick_light(const float *cdf, int nl, float u) {
 	int lo = 0, hi = nl - 1;
 	while (lo < hi) {
 		int mid = (lo + hi) >> 1;
-		if (u < cdf[mid])
+		if (u < gload_f(cdf + mid))
 			hi = mid;
 		else
 			lo = mid + 1;
@@ -47,7 +47,10 @@ RN_DEV int pick_light(const float *cdf, int nl, float u) {
 	return lo;
 }
 
-RN_DEV float3 load3(const float *p, int i) { return f3(p[3 * i], p[3 * i + 1], p[3 * i + 2]); }
+// Scene buffers are global memory (see gload_f)
+RN_DEV float3 load3(const float *p, int i) {
+	return f3(gload_f(p + 3 * i), gload_f(p + 3 * i + 1), gload_f(p + 3 * i + 2));
+}
 
 // Geometric normal of leaf-ordered triangle `leaf`, facing against d.
 // Unit geometric normal facing against d, from cross(v1 - v0, v2 - v0).
@@ -73,7 +76,8 @@ struct NeeSample {
 
 RN_DEV bool sample_light(const DevScene &sc, float3 so, float3 ng, float ul, float u1, float u2, NeeSample &ns) {
 	int li = pick_light(sc.light_cdf, sc.nl, ul);
-	float4 g0 = sc.light_geom[3 * li], g1 = sc.light_geom[3 * li + 1], g2 = sc.light_geom[3 * li + 2];
+	float4 g0 = gload_f4(sc.light_geom, 3 * li), g1 = gload_f4(sc.light_geom, 3 * li + 1),
+	       g2 = gload_f4(sc.light_geom, 3 * li + 2);
 	float su = sqrtf(u1);
 	float b1 = su * (1.0f - u2), b2 = su * u2;
 	float3 e1 = xyz(g1), e2 = xyz(g2);
@@ -90,7 +94,7 @@ RN_DEV bool sample_light(const DevScene &sc, float3 so, float3 ng, float ul, flo
 	ns.wi = wi;
 	ns.tmax = dist * 0.9999f;
 	ns.Y = cs * cl / d2 * sc.total_light_area * RN_INV_PI;
-	ns.light = sc.light_tri[li];
+	ns.light = gload_i(sc.light_tri + li);
 	return true;
 }
 

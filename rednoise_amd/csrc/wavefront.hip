@@ -384,9 +384,9 @@ RN_DEV void trace_queue(const WQ &q, int cur, uint64_t *stk, const uint8_t *octa
 				const float4 c4 = q.sc[idx];
 				float *out = ONE ? out_one : q.scenes[__float_as_uint(c4.w)].out;
 				float *fb = out + 3 * (size_t)__float_as_uint(q.so[idx].w);
-				atomicAdd(fb, c4.x);
-				atomicAdd(fb + 1, c4.y);
-				atomicAdd(fb + 2, c4.z);
+				atomic_add_global(fb, c4.x);
+				atomic_add_global(fb + 1, c4.y);
+				atomic_add_global(fb + 2, c4.z);
 			}
 		}
 	}
@@ -490,9 +490,9 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(WQ q, int cur) {
 					float3 le = load3(sc.emission, prim);
 					if (le.x != 0.f || le.y != 0.f || le.z != 0.f) {
 						float *fb = ws.out + 3 * (size_t)pix;
-						atomicAdd(fb, beta.x * le.x);
-						atomicAdd(fb + 1, beta.y * le.y);
-						atomicAdd(fb + 2, beta.z * le.z);
+						atomic_add_global(fb, beta.x * le.x);
+						atomic_add_global(fb + 1, beta.y * le.y);
+						atomic_add_global(fb + 2, beta.z * le.z);
 					}
 				}
 				float3 a = load3(sc.albedo, prim);
@@ -571,10 +571,20 @@ __global__ void k_resolve(float *out, size_t n, float scale) {
 struct WaveWork {
 	DevBuf state, hit, shadow, ctl, scenes;
 	unsigned *pinned_flag = nullptr; // host-pinned done, iter, done_iter; kept between renders
-	// The instantiated graph of the last graph render, with the launch
-	// arguments it has baked in; reused while they stay the same.
-	hipGraphExec_t exec = nullptr;
-	std::vector<uint64_t> exec_key;
+	// Instantiated graphs of the last graph render, with the launch arguments
+	// and iteration counts they have baked in; reused while those stay the
+	// same. `group` holds a group of iterations, `first` the iterations a
+	// render enqueues before it first reads the done flag.
+	struct Cached {
+		hipGraphExec_t exec = nullptr;
+		std::vector<uint64_t> key;
+		void drop() {
+			if (exec)
+				(void)hipGraphExecDestroy(exec);
+			exec = nullptr;
+			key.clear();
+		}
+	} group, first;
 	// Iterations the last render needed, with its launch arguments: a render
 	// with the same ones enqueues that many before it first reads the flag.
 	std::vector<uint64_t> pred_key;
@@ -602,10 +612,9 @@ WaveWork &work() {
 
 void wavefront_release() {
 	WaveWork &ww = work();
-	if (ww.exec)
-		(void)hipGraphExecDestroy(ww.exec); // it refers to the queues freed below
-	ww.exec = nullptr;
-	ww.exec_key.clear();
+	// the graphs refer to the queues freed below
+	ww.group.drop();
+	ww.first.drop();
 	ww.pred_key.clear();
 	ww.state.release();
 	ww.hit.release();
@@ -864,27 +873,27 @@ void wavefront_render_on(const WaveItem *items, int count, const WaveConfig &cfg
 		key.push_back(((uint64_t)deep << 2) | ((uint64_t)(count == 1) << 1) | (uint64_t)cfg.use_graph);
 		key.push_back((uint64_t)per);
 	}
-	hipGraphExec_t exec = nullptr;
-	if (cfg.use_graph) {
-		if (ww.exec && ww.exec_key == key) {
-			exec = ww.exec;
-		} else {
-			if (ww.exec)
-				RN_HIP(hipGraphExecDestroy(ww.exec));
-			ww.exec = nullptr;
-			ww.exec_key.clear();
-			hipGraph_t graph = nullptr;
-			RN_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-			for (int it = 0; it < per; it++)
-				iteration(it & 1);
-			RN_HIP(hipStreamEndCapture(s, &graph));
-			hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-			(void)hipGraphDestroy(graph);
-			RN_HIP(e);
-			ww.exec = exec;
-			ww.exec_key = key;
-		}
-	}
+	// A graph of n iterations (n even: every iteration swaps the path
+	// queues, so a graph starts and ends on the same one), cached in c.
+	auto graph_of = [&](WaveWork::Cached &c, uint64_t n) {
+		std::vector<uint64_t> k = key;
+		k.push_back(n);
+		if (c.exec && c.key == k)
+			return c.exec;
+		c.drop();
+		hipGraph_t graph = nullptr;
+		RN_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+		for (uint64_t it = 0; it < n; it++)
+			iteration((int)(it & 1));
+		RN_HIP(hipStreamEndCapture(s, &graph));
+		hipGraphExec_t e = nullptr;
+		const hipError_t rc = hipGraphInstantiate(&e, graph, nullptr, nullptr, 0);
+		(void)hipGraphDestroy(graph);
+		RN_HIP(rc);
+		c.exec = e;
+		c.key = std::move(k);
+		return e;
+	};
 	uint64_t iters = 0;
 	// Every iteration retires or advances each queued path, so the loop
 	// ends within (batches of queue refills) x (max depth + 2) iterations.
@@ -893,24 +902,24 @@ void wavefront_render_on(const WaveItem *items, int count, const WaveConfig &cfg
 		maxd = std::max(maxd, items[i].params.max_depth);
 	uint64_t limit = ((total + cap - 1) / cap + 1) * (uint64_t)(maxd + 2) + 64;
 	if (fault_injected("wavefront_iterations"))
-		limit = 0; // the check below fires after the first launch group
-	// groups of `per` iterations enqueued before the flag is first read
-	uint64_t groups = 1;
+		limit = 0; // the check below fires after the first launch
+	// Iterations enqueued before the done flag is first read: as many as the
+	// last render with the same launch arguments needed (rounded up to even),
+	// else one group; after that, one group per read of the flag.
+	uint64_t n = (uint64_t)per;
 	if (ww.pred_key == key && ww.pred_iters > 0)
-		groups = (ww.pred_iters + (uint64_t)per - 1) / (uint64_t)per;
+		n = ww.pred_iters + (ww.pred_iters & 1);
 	while (true) {
 		if (iters > limit)
 			throw Error("wavefront: iteration limit exceeded (queue accounting error)");
-		for (uint64_t g = 0; g < groups; g++) {
-			if (cfg.use_graph) {
-				RN_HIP(hipGraphLaunch(exec, s));
-			} else {
-				for (int it = 0; it < per; it++)
-					iteration(it & 1);
-			}
-			iters += (uint64_t)per;
+		if (cfg.use_graph) {
+			RN_HIP(hipGraphLaunch(iters == 0 ? graph_of(ww.first, n) : graph_of(ww.group, n), s));
+		} else {
+			for (uint64_t it = 0; it < n; it++)
+				iteration((int)(it & 1));
 		}
-		groups = 1;
+		iters += n;
+		n = (uint64_t)per;
 		RN_HIP(hipGetLastError());
 		RN_HIP(hipMemcpyAsync(flag, &q.ctl->done, 3 * sizeof(unsigned), hipMemcpyDeviceToHost, s));
 		RN_HIP(hipStreamSynchronize(s));
@@ -922,7 +931,7 @@ void wavefront_render_on(const WaveItem *items, int count, const WaveConfig &cfg
 	ww.pred_key = key;
 	ww.pred_iters = needed;
 	st.iterations = needed;
-	st.extend_launches = needed;
+	st.extend_launches = iters;
 	if (timed) {
 		double acc[3] = {0, 0, 0};
 		for (size_t k = 0; k < stage_of.size(); k++) {

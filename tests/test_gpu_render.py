@@ -116,6 +116,48 @@ def test_graph_replays_match_eager(rn):
     assert np.allclose(so.render_hdr(cam, g), so.render_hdr(cam, base), rtol=1e-5, atol=1e-6)
 
 
+@pytest.mark.parametrize("graph", [0, 1])
+def test_first_launch_follows_the_last_render(rn, graph):
+    """A wavefront render enqueues as many iterations as the last render with
+    the same launch arguments needed (rounded up to even) before it first
+    reads the done flag. Too many: the extra iterations find the queues empty.
+    Too few: it goes on in groups. The images equal the megakernel's either
+    way. Roulette depth lives in the scene table, not in the launch
+    arguments, so it changes the iterations a render needs under one key."""
+    from rednoise_amd import scenes
+    sd = scenes.cornell_box()
+    sc = rn.Scene.from_arrays(sd.verts, sd.albedo, sd.emission)
+    cam = rn.Camera.make(**sd.camera)
+
+    def params(rr, integ):
+        return rn.RenderParams.make(80, 60, 16, 12, rr, seed=4, ray_eps=0.05, integrator=integ, queue_paths=8192,
+                                    flags=rn.FLAG_USE_GRAPH if graph and integ == rn.INTEGRATOR_WAVEFRONT else 0)
+
+    want = {rr: sc.render_hdr(cam, params(rr, rn.INTEGRATOR_MEGAKERNEL)) for rr in (1, 1 << 30)}
+    needed = {}
+
+    def render(rr):
+        img = sc.render_hdr(cam, params(rr, rn.INTEGRATOR_WAVEFRONT))
+        st = rn.last_stats()
+        assert np.allclose(img, want[rr], rtol=1e-4, atol=1e-5)
+        assert st["trace_kernel_launches"] >= st["iterations"]
+        needed.setdefault(rr, st["iterations"])
+        assert st["iterations"] == needed[rr]
+        return st["trace_kernel_launches"]
+
+    render(1 << 30)
+    long_n = needed[1 << 30]
+    assert render(1 << 30) == long_n + (long_n & 1)  # predicted exactly
+    render(1)  # predicted too many
+    short_n = needed[1]
+    assert short_n < long_n, (short_n, long_n)
+    assert render(1) == short_n + (short_n & 1)
+    launched = render(1 << 30)  # predicted too few: goes on in groups
+    assert launched >= long_n and (launched - (short_n + (short_n & 1))) % 8 == 0
+    assert render(1 << 30) == long_n + (long_n & 1)
+    sc.close()
+
+
 def test_bvh_width_4_matches_8(rn):
     from rednoise_amd import scenes
     sd = scenes.sphere_scene_tris(8000, seed=3)
