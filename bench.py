@@ -121,7 +121,11 @@ def adjoint_bench(rn, lib, torch, n_fd=64):
     sct = rn.Scene.from_arrays(tgt.verts, tgt.albedo, tgt.emission)
     pt = rn.RenderParams.make(W, H, spp, depth, rr_depth=1 << 30, seed=1234, ray_eps=0.03)
     target = sct.render_hdr(cam, pt).astype(np.float64)
-    p = rn.RenderParams.make(W, H, spp, depth, rr_depth=1 << 30, seed=0, ray_eps=0.03)
+    # forward pass: the wavefront integrator (the estimator, and the Philox
+    # streams the adjoint replays, are those of the megakernel, timed beside it)
+    p = rn.RenderParams.make(W, H, spp, depth, rr_depth=1 << 30, seed=0, ray_eps=0.03,
+                             integrator=rn.INTEGRATOR_WAVEFRONT)
+    pm = rn.RenderParams.make(W, H, spp, depth, rr_depth=1 << 30, seed=0, ray_eps=0.03)
     n = sc.triangle_count
     img = sc.render_hdr(cam, p)
     dL = (img.astype(np.float64) - target).astype(np.float32)  # L = 0.5 sum (I - T)^2
@@ -142,8 +146,16 @@ def adjoint_bench(rn, lib, torch, n_fd=64):
     kern_adj = st["render_ms"]
     sc.render_hdr(cam, p)
     kern_fwd = rn.last_stats()["render_ms"]
+    sc.render_hdr(cam, pm)
+    t0 = time.perf_counter()
+    img_m = sc.render_hdr(cam, pm)
+    fwd_mega_ms = 1e3 * (time.perf_counter() - t0)
+    same_image = bool(np.allclose(img_m, img, rtol=1e-4, atol=1e-5))
     # central differences on 64 random sphere triangles (albedo, one channel
-    # each), loss L = 0.5 sum (I - T)^2 evaluated in FP64 from the FP32 images.
+    # each), loss L = 0.5 sum (I - T)^2 evaluated in FP64 from the FP32 images,
+    # of the megakernel's forward pass: it writes each pixel once, so two
+    # renders differ only where the perturbed albedo enters (the wavefront adds
+    # into pixels with atomics, whose order varies between runs).
     rng = np.random.default_rng(64)
     tris = rng.choice(np.arange(12, n), size=n_fd, replace=False)
     chans = rng.integers(0, 3, size=n_fd)
@@ -155,11 +167,11 @@ def adjoint_bench(rn, lib, torch, n_fd=64):
         a0 = alb[t, c]
         alb[t, c] = a0 + h
         sc.set_materials(albedo=alb)
-        ip = sc.render_hdr(cam, p).astype(np.float64)
+        ip = sc.render_hdr(cam, pm).astype(np.float64)
         hi = float(alb[t, c])
         alb[t, c] = a0 - h
         sc.set_materials(albedo=alb)
-        im = sc.render_hdr(cam, p).astype(np.float64)
+        im = sc.render_hdr(cam, pm).astype(np.float64)
         lo = float(alb[t, c])
         alb[t, c] = a0
         dloss = 0.5 * float(((ip - im) * (ip + im - 2.0 * target)).sum())
@@ -172,6 +184,8 @@ def adjoint_bench(rn, lib, torch, n_fd=64):
     return {
         "workload": f"config3: {n} tris (cube sphere m=91 + 12 Cornell), 512x512, 32 spp, depth 6, no roulette",
         "ms_per_iter": round(fwd_ms + adj_ms, 3), "forward_ms": round(fwd_ms, 3), "adjoint_ms": round(adj_ms, 3),
+        "forward_integrator": "wavefront", "forward_megakernel_ms": round(fwd_mega_ms, 3),
+        "forward_images_equal": same_image,
         "adjoint_over_forward": round(adj_ms / fwd_ms, 3),
         "kernel_adjoint_over_forward": round(kern_adj / kern_fwd, 3),
         "grad_rel_err_vs_fd": err, "fd_h": h, "fd_triangles": n_fd,

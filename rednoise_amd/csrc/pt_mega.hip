@@ -12,6 +12,10 @@ namespace {
 constexpr int kMBlock = 128; // 16 x 8 pixel tile, two waves
 constexpr int kTileW = 16, kTileH = 8;
 constexpr int kMaxAdjDepth = 16;
+#ifndef RN_ADJ_WAVES_PER_CU
+#define RN_ADJ_WAVES_PER_CU 256
+#endif
+constexpr int kAdjWavesPerCU = RN_ADJ_WAVES_PER_CU;
 
 struct AdjRec {
 	int tri;
@@ -153,10 +157,13 @@ __global__ __launch_bounds__(kMBlock) void k_mega(DevScene sc, DevCamera cam, De
 	}
 }
 
+// Block (tile, g) replays samples [g * spl, (g + 1) * spl) of its tile's
+// pixels: the gradients are sums of atomics, so a pixel's samples may be
+// spread over blocks, and more, shorter blocks balance the load.
 template <int LDS, int SPILL>
 __global__ __launch_bounds__(kMBlock) void k_adjoint(DevScene sc, DevCamera cam, DevParams p, const float *dl_dimage,
                                                      const int *hot_slot, const int *hot_prim, int n_hot,
-                                                     float *g_albedo, float *g_emission) {
+                                                     float *g_albedo, float *g_emission, int spl) {
 	__shared__ uint64_t stk[LDS * kMBlock];
 	__shared__ float la[3 * kHotSlots], le[3 * kHotSlots];
 	for (int i = threadIdx.x; i < 3 * kHotSlots; i += kMBlock) {
@@ -174,8 +181,9 @@ __global__ __launch_bounds__(kMBlock) void k_adjoint(DevScene sc, DevCamera cam,
 		float3 w = f3(dl_dimage[pi] * inv, dl_dimage[pi + 1] * inv, dl_dimage[pi + 2] * inv);
 		GradSink gs{g_albedo, g_emission, la, le, hot_slot};
 		unsigned long long ncl = 0, nsh = 0;
+		const int s0 = (int)blockIdx.y * spl, s1 = min(p.spp, s0 + spl);
 		if (w.x != 0.f || w.y != 0.f || w.z != 0.f)
-			for (int s = 0; s < p.spp; s++)
+			for (int s = s0; s < s1; s++)
 				trace_sample<LDS, SPILL, true>(sc, cam, p, x, y, p.sample_base + (uint32_t)s, stk, threadIdx.x, ncl, nsh, w, &gs);
 	}
 	__syncthreads();
@@ -218,12 +226,22 @@ void launch_adjoint(const DevScene &sc, const DevCamera &cam, const DevParams &p
 		throw Error("rn_adjoint supports max_depth <= 16");
 	if (n_hot > kHotSlots)
 		throw Error("too many hot triangles");
+	// Sample groups: enough blocks for about kAdjWavesPerCU waves per CU
+	// (16 are resident at a time), each group at least one sample.
+	int dev = 0, n_cu = 0;
+	RN_HIP(hipGetDevice(&dev));
+	RN_HIP(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
+	const long long waves = (long long)mega_grid(p) * (kMBlock / 64);
+	const long long want = (long long)std::max(1, n_cu) * kAdjWavesPerCU;
+	const int groups = (int)std::min<long long>(std::max(1, p.spp), std::max<long long>(1, (want + waves - 1) / waves));
+	const int spl = (p.spp + groups - 1) / groups;
+	const dim3 grid(mega_grid(p), (unsigned)((p.spp + spl - 1) / spl));
 	if (sc.depth - 1 > kLdsDepth)
 		k_adjoint<kLdsDepth, kSpillDepth>
-		    <<<mega_grid(p), kMBlock, 0, s>>>(sc, cam, p, dl_dimage, hot_slot, hot_prim, n_hot, g_albedo, g_emission);
+		    <<<grid, kMBlock, 0, s>>>(sc, cam, p, dl_dimage, hot_slot, hot_prim, n_hot, g_albedo, g_emission, spl);
 	else
 		k_adjoint<kLdsDepth, 0>
-		    <<<mega_grid(p), kMBlock, 0, s>>>(sc, cam, p, dl_dimage, hot_slot, hot_prim, n_hot, g_albedo, g_emission);
+		    <<<grid, kMBlock, 0, s>>>(sc, cam, p, dl_dimage, hot_slot, hot_prim, n_hot, g_albedo, g_emission, spl);
 	RN_HIP(hipGetLastError());
 }
 
