@@ -107,7 +107,12 @@ typedef struct {
 enum {
 	RN_FLAG_COUNT_TRAVERSAL = 1, /* megakernel: also count BVH nodes / triangles per ray type (slower) */
 	RN_FLAG_TIME_STAGES = 2,     /* wavefront: time every stage launch with HIP events */
-	RN_FLAG_USE_GRAPH = 4        /* wavefront: replay the iteration from a captured hipGraph */
+	RN_FLAG_USE_GRAPH = 4,       /* wavefront: replay the iteration from a captured hipGraph */
+	RN_FLAG_FB_ATOMICS = 8       /* wavefront: add contributions to the frame buffer with float atomics
+	                              * (arbitrary order) instead of per-path radiance slots summed in sample
+	                              * order; slots are also left out when the render's paths number 2^32 or
+	                              * more or their 12 bytes each exceed a third of the free device memory,
+	                              * and by rn_render_ex_device_noalloc */
 };
 
 /* Physical ("Mode P") estimator: Lambertian albedo/pi, next-event estimation to

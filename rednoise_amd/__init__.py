@@ -33,6 +33,7 @@ INTEGRATOR_WAVEFRONT = 1
 FLAG_COUNT_TRAVERSAL = 1
 FLAG_TIME_STAGES = 2
 FLAG_USE_GRAPH = 4
+FLAG_FB_ATOMICS = 8
 QUERY_CLOSEST = 0
 QUERY_ANY = 1
 TRI_WATERTIGHT = 0

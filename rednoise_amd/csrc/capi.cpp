@@ -750,6 +750,7 @@ static int render_common(const rn_scene *scene, const rn_camera *camera, const r
 		cfg.block = params->block_size;
 		cfg.time_stages = (params->flags & RN_FLAG_TIME_STAGES) != 0;
 		cfg.use_graph = (params->flags & RN_FLAG_USE_GRAPH) != 0;
+		cfg.fb_atomics = (params->flags & RN_FLAG_FB_ATOMICS) != 0;
 		wavefront_render(&item, 1, cfg, cnt, s, ws);
 	} else {
 		launch_mega(ds, cam, p, out, cnt, (params->flags & RN_FLAG_COUNT_TRAVERSAL) != 0, s);

@@ -42,6 +42,14 @@ RN_DEV void grad_add(float *glob, float *lds, const int *hot_slot, int prim, flo
 	}
 }
 
+// L + c with each sum rounded on its own (no fused multiply-add with the
+// product that made c): the wavefront integrator adds a stored contribution
+// to a path's radiance slot, so both integrators round alike.
+RN_DEV float3 add_uncontracted(float3 L, float3 c) {
+#pragma clang fp contract(off)
+	return f3(L.x + c.x, L.y + c.y, L.z + c.z);
+}
+
 // One camera path of the physical estimator. With ADJ the path is a replay:
 // emission derivatives are pushed as they appear and the albedo derivatives
 // by the reverse sweep over the vertex records at the end.
@@ -82,7 +90,7 @@ RN_DEV float3 trace_sample(const DevScene &sc, const DevCamera &cam, const DevPa
 				if (sh.prim == 0) {
 					float3 le = load3(sc.emission, ns.light);
 					X = le * ns.Y;
-					L = L + beta * a * X;
+					L = add_uncontracted(L, beta * a * X);
 					if (ADJ)
 						grad_add(gs->g_emission, gs->l_emission, gs->hot_slot, ns.light, wgt * beta * a * ns.Y);
 				}

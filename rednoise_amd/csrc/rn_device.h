@@ -189,8 +189,11 @@ RN_DEV void intersect_children(const uint32_t w[20], const TRay &r, float tmax, 
 // Watertight test (Woop, Benthin, Wald, JCGT 2(1) 2013). Ray-constant shear
 // terms are computed once per ray.
 struct WRay {
-	int kx, ky, kz;
+	uint32_t k; // the axes kx | ky << 2 | kz << 4, packed: one register per ray instead of three
 	float sx, sy, sz;
+	RN_DEV int kx() const { return (int)(k & 3u); }
+	RN_DEV int ky() const { return (int)((k >> 2) & 3u); }
+	RN_DEV int kz() const { return (int)(k >> 4); }
 };
 
 RN_DEV WRay make_wray(float3 d) {
@@ -204,9 +207,7 @@ RN_DEV WRay make_wray(float3 d) {
 		kx = ky;
 		ky = t;
 	}
-	w.kx = kx;
-	w.ky = ky;
-	w.kz = kz;
+	w.k = (uint32_t)kx | ((uint32_t)ky << 2) | ((uint32_t)kz << 4);
 	float dz = get3(d, kz);
 	w.sx = get3(d, kx) / dz;
 	w.sy = get3(d, ky) / dz;
@@ -219,10 +220,11 @@ RN_DEV bool tri_watertight(const WRay &w, float3 o, float3 v0, float3 v1, float3
                            float &v) {
 #pragma clang fp contract(off)
 	float3 A = v0 - o, B = v1 - o, C = v2 - o;
-	float Az0 = get3(A, w.kz), Bz0 = get3(B, w.kz), Cz0 = get3(C, w.kz);
-	float Ax = get3(A, w.kx) - w.sx * Az0, Ay = get3(A, w.ky) - w.sy * Az0;
-	float Bx = get3(B, w.kx) - w.sx * Bz0, By = get3(B, w.ky) - w.sy * Bz0;
-	float Cx = get3(C, w.kx) - w.sx * Cz0, Cy = get3(C, w.ky) - w.sy * Cz0;
+	const int kx = w.kx(), ky = w.ky(), kz = w.kz();
+	float Az0 = get3(A, kz), Bz0 = get3(B, kz), Cz0 = get3(C, kz);
+	float Ax = get3(A, kx) - w.sx * Az0, Ay = get3(A, ky) - w.sy * Az0;
+	float Bx = get3(B, kx) - w.sx * Bz0, By = get3(B, ky) - w.sy * Bz0;
+	float Cx = get3(C, kx) - w.sx * Cz0, Cy = get3(C, ky) - w.sy * Cz0;
 	float U = Cx * By - Cy * Bx, V = Ax * Cy - Ay * Cx, W = Bx * Ay - By * Ax;
 	if (U == 0.0f || V == 0.0f || W == 0.0f) {
 		// Edge functions recomputed exactly (float products are exact in

@@ -20,6 +20,7 @@ struct WaveConfig {
 	bool use_graph = false;  // capture the iteration loop in a hipGraph
 	int iters_per_graph = 0; // <= 0: 8 (rounded up to even)
 	bool time_stages = false;
+	bool fb_atomics = false; // contributions by frame-buffer float atomics instead of per-path radiance slots
 };
 
 struct WaveStats {

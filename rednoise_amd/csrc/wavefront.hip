@@ -5,9 +5,10 @@
 // Per iteration four kernels run:
 //   generate - appends freshly generated camera paths to the queue;
 //   trace    - closest hit of every queued path (SoA state), then the any-hit
-//              rays the previous iteration's shade stage pushed (unoccluded
-//              contributions are added to the frame buffer with float
-//              atomics);
+//              rays the previous iteration's shade stage pushed (an
+//              unoccluded ray stores its path's new radiance in the path's
+//              slot; without slots it adds its contribution to the frame
+//              buffer with float atomics);
 //   shade    - emission, NEE light sample, bounce and Russian roulette;
 //              shadow rays and continuing paths are pushed with one wave64
 //              ballot per wave and one atomic per block (compaction keeps
@@ -87,6 +88,16 @@ struct WQ {
 	float4 *hitn;  // per queue entry: cross(v1 - v0, v2 - v0) of the hit triangle, t
 	float4 *so, *sd, *sc;
 	WCtl *ctl;
+	// Radiance slots, 3 floats per path of the render's work range (global
+	// work index = slot), or null: contributions go to the frame buffers
+	// with float atomics. A path's depth-0 shade stage writes its slot (the
+	// camera ray's emission, or zero); a shade stage that pushes an any-hit
+	// ray reads the slot and puts its value plus the ray's contribution in
+	// the ray's record, and the next trace launch stores that if the ray is
+	// unoccluded (one any-hit ray per path per launch, and only shade stages
+	// and trace launches between them touch the slot, so no write races);
+	// k_reduce_slots sums a pixel's slots in sample order.
+	float *acc;
 	const WScene *scenes;
 	int nscenes;
 	int max_depth; // the largest of the scenes' max_depth (paths live at most this many iterations)
@@ -218,6 +229,52 @@ RN_DEV unsigned lanes_below(uint64_t m) {
 // meta word: scene index (low 16 bits) | depth << 16; depth 255 = dead slot.
 constexpr uint32_t kDead = 255u;
 
+// Radiance slot of the path of pixel `pix`, sample `smp`: its global work
+// index, i.e. the inverse of k_generate's order (sample-major, 8x8 tiles,
+// rows within a tile). A camera path's slot is the work index k_generate
+// gave its queue entry; later vertices find it from the pixel and sample
+// they carry. The host keeps the work range below 2^32 in slot mode.
+struct SlotMap {
+	uint32_t first, width, tiles_x, npix_pad, sbase;
+};
+RN_DEV SlotMap slot_map(const WScene &ws) {
+	return SlotMap{(uint32_t)ws.first, (uint32_t)ws.p.width, (uint32_t)ws.tiles_x, (uint32_t)ws.npix_pad,
+	               ws.p.sample_base};
+}
+// The map of a scene the whole launch works on, read once into scalar
+// registers (a load inside the shade loop would be a vector load: the
+// loop's stores could alias the scene table as far as the compiler knows).
+RN_DEV SlotMap slot_map_uniform(const WScene &ws) {
+	const SlotMap m = slot_map(ws);
+	return SlotMap{(uint32_t)__builtin_amdgcn_readfirstlane(m.first), (uint32_t)__builtin_amdgcn_readfirstlane(m.width),
+	               (uint32_t)__builtin_amdgcn_readfirstlane(m.tiles_x),
+	               (uint32_t)__builtin_amdgcn_readfirstlane(m.npix_pad), (uint32_t)__builtin_amdgcn_readfirstlane(m.sbase)};
+}
+RN_DEV uint32_t path_slot(const SlotMap &m, uint32_t pix, uint32_t smp) {
+	const uint32_t y = pix / m.width, x = pix - y * m.width;
+	const uint32_t pp = ((((y >> 3) * m.tiles_x) + (x >> 3)) << 6) | ((y & 7u) << 3) | (x & 7u);
+	return m.first + (smp - m.sbase) * m.npix_pad + pp;
+}
+
+// a + b rounded on its own, never fused with the product that made b (the
+// megakernel adds its contributions the same way).
+RN_DEV float3 add_uncontracted(float3 a, float3 b) {
+#pragma clang fp contract(off)
+	return f3(a.x + b.x, a.y + b.y, a.z + b.z);
+}
+
+// Plain loads and stores of a radiance slot (global memory).
+RN_DEV float3 slot_load(const float *p) {
+	typedef __attribute__((address_space(1))) const float gf;
+	return f3(((gf *)p)[0], ((gf *)p)[1], ((gf *)p)[2]);
+}
+RN_DEV void slot_store(float *p, float3 v) {
+	typedef __attribute__((address_space(1))) float gf;
+	((gf *)p)[0] = v.x;
+	((gf *)p)[1] = v.y;
+	((gf *)p)[2] = v.z;
+}
+
 // New camera paths for queue entries [cur_count, cur_count + gen_count).
 __global__ __launch_bounds__(256) void k_generate(WQ q, int cur) {
 	WCtl *c = q.ctl;
@@ -306,6 +363,12 @@ RN_DEV void trace_queue(const WQ &q, int cur, uint64_t *stk, const uint8_t *octa
 	unsigned next = 0, end = 0;
 	bool more = true, valid = false;
 	unsigned idx = 0;
+	// Any hit, read at the refill so that an unoccluded ray's write waits
+	// for no load: the ray's radiance slot and the slot's value with the
+	// ray's contribution added (or its pixel and contribution without
+	// slots), and (batches) its scene.
+	unsigned aux = 0, fin_scene = 0;
+	float3 fin_c = f3(0.f, 0.f, 0.f);
 	// ONE: every queued ray belongs to scene c->single, whose BVH, triangles
 	// and frame buffer are held in scalar registers
 	const int one = ONE ? c->single : -1;
@@ -352,7 +415,10 @@ RN_DEV void trace_queue(const WQ &q, int cur, uint64_t *stk, const uint8_t *octa
 					// path records stream past non-temporally; shadow records
 					// load plainly (0.26 ms per frame faster than streamed)
 					const float4 o4 = QUERY == 0 ? get4(O + my) : O[my], d4 = QUERY == 0 ? get4(D + my) : D[my];
-					const uint32_t meta = (ONE && QUERY == 1) ? 0u : __float_as_uint(M[my].w);
+					// closest hit: the path's meta word; any hit: the
+					// contribution and scene index
+					const float4 m4 = QUERY == 1 ? M[my] : make_float4(0.f, 0.f, 0.f, M[my].w);
+					const uint32_t meta = __float_as_uint(m4.w);
 					if (QUERY == 1 || (meta >> 16) != kDead) {
 						if (!ONE) {
 							const DevScene &sc = q.scenes[meta & 0xFFFFu].sc;
@@ -362,6 +428,12 @@ RN_DEV void trace_queue(const WQ &q, int cur, uint64_t *stk, const uint8_t *octa
 						trav_init<0>(t, xyz(o4), xyz(d4), QUERY == 0 ? INFINITY : d4.w);
 						valid = true;
 						idx = my;
+						if (QUERY == 1) {
+							aux = __float_as_uint(o4.w);
+							fin_c = xyz(m4);
+							if (!ONE)
+								fin_scene = meta & 0xFFFFu;
+						}
 						nrays++;
 					}
 				}
@@ -381,12 +453,15 @@ RN_DEV void trace_queue(const WQ &q, int cur, uint64_t *stk, const uint8_t *octa
 				if (t.h.prim >= 0)
 					put4m(q.hitn + idx, make_float4(t.h.ng.x, t.h.ng.y, t.h.ng.z, t.h.t));
 			} else if (t.h.prim == 0) {
-				const float4 c4 = q.sc[idx];
-				float *out = ONE ? out_one : q.scenes[__float_as_uint(c4.w)].out;
-				float *fb = out + 3 * (size_t)__float_as_uint(q.so[idx].w);
-				atomic_add_global(fb, c4.x);
-				atomic_add_global(fb + 1, c4.y);
-				atomic_add_global(fb + 2, c4.z);
+				if (q.acc) {
+					slot_store(q.acc + 3 * (size_t)aux, fin_c);
+				} else {
+					float *out = ONE ? out_one : q.scenes[fin_scene].out;
+					float *fb = out + 3 * (size_t)aux;
+					atomic_add_global(fb, fin_c.x);
+					atomic_add_global(fb + 1, fin_c.y);
+					atomic_add_global(fb + 2, fin_c.z);
+				}
 			}
 		}
 	}
@@ -445,15 +520,25 @@ __global__ __launch_bounds__(BS) RN_TRACE_ATTR void k_trace(WQ q, int cur) {
 constexpr int kShadeBlock = RN_SHADE_BLOCK;
 constexpr int kShadeWaves = kShadeBlock / 64;
 
+// Two blocks per CU (8 waves per SIMD) need at most 64 VGPRs and 80 SGPRs:
+// one SGPR more halves the single-scene instance's occupancy (shade 17-25%
+// slower per iteration at 1M triangles, 1080p), so the occupancy is asked
+// for. The batch instance, at 74 VGPRs, runs one block per CU either way.
 template <bool ONE>
-__global__ __launch_bounds__(kShadeBlock) void k_shade(WQ q, int cur) {
+__global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(ONE ? 8 : 0, ONE ? 8 : 0))) void k_shade(
+    WQ q, int cur) {
 	if (instance_skips<ONE>(q))
 		return;
 	const unsigned one = ONE ? (unsigned)q.ctl->single : 0u;
-	__shared__ unsigned wcnt[2][kShadeWaves];
+	const SlotMap m1 = ONE && q.acc ? slot_map_uniform(q.scenes[one]) : SlotMap{};
+	static_assert(kShadeWaves <= 64 && kShadeBlock < 65536, "block counts are packed in 16-bit halves");
+	__shared__ unsigned wcnt[kShadeWaves]; // per wave: shadow pushes | continuing pushes << 16
 	__shared__ unsigned bbase[2];
 	WCtl *c = q.ctl;
-	const unsigned n = c->cur_count + c->gen_count;
+	// entries [cur_count, cur_count + gen_count) are the camera paths of
+	// work indices gen_base + (i - cur_count)
+	const unsigned ncur = c->cur_count, n = ncur + c->gen_count;
+	const unsigned long long gbase = c->gen_base;
 	const float4 *O = q.o[cur], *D = q.d[cur], *B = q.b[cur];
 	float4 *NO = q.o[cur ^ 1], *ND = q.d[cur ^ 1], *NB = q.b[cur ^ 1];
 	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -471,6 +556,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(WQ q, int cur) {
 		float3 so = f3(0.f, 0.f, 0.f), wo = so, beta = so, contrib = so, shw = so;
 		float shmax = 0.f;
 		float4 o4 = make_float4(0.f, 0.f, 0.f, 0.f), d4 = o4;
+		uint32_t aux = 0; // slot mode: the path's radiance slot
 		if (depth != kDead) {
 			const int prim = geti(q.hitp + i);
 			if (prim >= 0) {
@@ -486,14 +572,26 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(WQ q, int cur) {
 				uint32_t pix = __float_as_uint(o4.w), smp = __float_as_uint(d4.w);
 				float3 P = o + d * hn.w;
 				float3 ng = facing_from_cross(xyz(hn), d);
+				// slot mode: the path's slot and its value once this vertex's
+				// emission is in (the any-hit ray pushed below carries that
+				// value plus its contribution, stored if it is unoccluded)
+				float3 sv = f3(0.f, 0.f, 0.f);
 				if (depth == 0) {
 					float3 le = load3(sc.emission, prim);
-					if (le.x != 0.f || le.y != 0.f || le.z != 0.f) {
+					if (q.acc) {
+						// the slot's first value: 0 + beta * le (the
+						// megakernel's first L + beta * le)
+						aux = (uint32_t)(gbase + (i - ncur));
+						sv = f3(beta.x * le.x, beta.y * le.y, beta.z * le.z);
+						slot_store(q.acc + 3 * (size_t)aux, sv);
+					} else if (le.x != 0.f || le.y != 0.f || le.z != 0.f) {
 						float *fb = ws.out + 3 * (size_t)pix;
 						atomic_add_global(fb, beta.x * le.x);
 						atomic_add_global(fb + 1, beta.y * le.y);
 						atomic_add_global(fb + 2, beta.z * le.z);
 					}
+				} else if (q.acc) {
+					aux = path_slot(ONE ? m1 : slot_map(ws), pix, smp);
 				}
 				float3 a = load3(sc.albedo, prim);
 				so = P + ng * p.ray_eps;
@@ -503,6 +601,8 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(WQ q, int cur) {
 					if (sample_light(sc, so, ng, u01(rn.x), u01(rn.y), u01(rn.z), ns)) {
 						float3 le = load3(sc.emission, ns.light);
 						contrib = beta * a * (le * ns.Y);
+						if (q.acc) // the slot's value: this vertex's emission at depth 0, else read here
+							contrib = add_uncontracted(depth == 0 ? sv : slot_load(q.acc + 3 * (size_t)aux), contrib);
 						shw = ns.wi;
 						shmax = ns.tmax;
 						push_sh = true;
@@ -521,37 +621,41 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(WQ q, int cur) {
 							beta = beta * (1.0f / qs);
 					}
 				}
+			} else if (depth == 0 && q.acc) {
+				// a camera ray that left the scene: its slot holds zero
+				slot_store(q.acc + 3 * (size_t)(uint32_t)(gbase + (i - ncur)), f3(0.f, 0.f, 0.f));
 			}
 		}
-		// block-wide compaction: wave counts -> one atomic per queue -> offsets
+		// Block-wide compaction: each wave's two push counts packed in one
+		// word (both at most 64, block sums below 2^16); thread 0 sums them
+		// and takes the block's entries with one 64-bit atomic; each wave adds
+		// the words of the waves before it.
 		const uint64_t msh = __ballot(push_sh), mnx = __ballot(push_next);
-		if (lane == 0) {
-			wcnt[0][wave] = (unsigned)__popcll(msh);
-			wcnt[1][wave] = (unsigned)__popcll(mnx);
-		}
+		if (lane == 0)
+			wcnt[wave] = (unsigned)__popcll(msh) | ((unsigned)__popcll(mnx) << 16);
 		__syncthreads();
 		if (threadIdx.x == 0) {
-			unsigned tsh = 0, tnx = 0;
-			for (int v = 0; v < kShadeWaves; v++) {
-				tsh += wcnt[0][v];
-				tnx += wcnt[1][v];
-			}
+			unsigned tot = 0;
+			for (int v = 0; v < kShadeWaves; v++)
+				tot += wcnt[v];
 			unsigned long long old = 0;
-			if (tsh | tnx)
+			if (tot)
 				old = atomicAdd(reinterpret_cast<unsigned long long *>(&c->next_count),
-				                (unsigned long long)tnx | ((unsigned long long)tsh << 32));
+				                (unsigned long long)(tot >> 16) | ((unsigned long long)(tot & 0xFFFFu) << 32));
 			bbase[0] = (unsigned)(old >> 32);
 			bbase[1] = (unsigned)old;
 		}
 		__syncthreads();
-		unsigned ss = bbase[0] + (unsigned)__popcll(msh & lt), ns = bbase[1] + (unsigned)__popcll(mnx & lt);
-		for (int v = 0; v < wave; v++) {
-			ss += wcnt[0][v];
-			ns += wcnt[1][v];
-		}
+		unsigned before = 0;
+		for (int v = 0; v < wave; v++)
+			before += wcnt[v];
+		const unsigned ss = bbase[0] + (before & 0xFFFFu) + (unsigned)__popcll(msh & lt);
+		const unsigned ns = bbase[1] + (before >> 16) + (unsigned)__popcll(mnx & lt);
 		__syncthreads(); // wcnt and bbase are rewritten by the next iteration
+		// the shadow record's .w: the path's radiance slot, or its pixel
+		// without slots
 		if (push_sh) {
-			put4(q.so + ss, make_float4(so.x, so.y, so.z, o4.w));
+			put4(q.so + ss, make_float4(so.x, so.y, so.z, q.acc ? __uint_as_float(aux) : o4.w));
 			put4(q.sd + ss, make_float4(shw.x, shw.y, shw.z, shmax));
 			put4(q.sc + ss, make_float4(contrib.x, contrib.y, contrib.z, __uint_as_float(meta & 0xFFFFu)));
 		}
@@ -568,8 +672,34 @@ __global__ void k_resolve(float *out, size_t n, float scale) {
 		out[i] *= scale;
 }
 
+// Each pixel of one scene: the sum of its paths' radiance slots in sample
+// order (from zero, as the megakernel sums its samples), times `scale`.
+// `acc` is the scene's first slot; a thread takes one tiled pixel index, so
+// a wave reads 64 consecutive slots per sample.
+__global__ void k_reduce_slots(float *out, const float *acc, int width, int height, int tiles_x, unsigned npix_pad,
+                               int spp, float scale) {
+	for (unsigned pp = blockIdx.x * blockDim.x + threadIdx.x; pp < npix_pad; pp += gridDim.x * blockDim.x) {
+		const unsigned tile = pp >> 6, t = pp & 63u;
+		const int x = (int)(tile % (unsigned)tiles_x) * kPixTile + (int)(t & 7u);
+		const int y = (int)(tile / (unsigned)tiles_x) * kPixTile + (int)(t >> 3);
+		if (x >= width || y >= height)
+			continue;
+		float3 sum = f3(0.f, 0.f, 0.f);
+		const float *a = acc + 3 * (size_t)pp;
+#pragma unroll 8
+		for (int s = 0; s < spp; s++) {
+			sum = sum + slot_load(a);
+			a += 3 * (size_t)npix_pad;
+		}
+		float *o = out + 3 * ((size_t)y * width + x);
+		o[0] = sum.x * scale;
+		o[1] = sum.y * scale;
+		o[2] = sum.z * scale;
+	}
+}
+
 struct WaveWork {
-	DevBuf state, hit, shadow, ctl, scenes;
+	DevBuf state, hit, shadow, ctl, scenes, acc;
 	unsigned *pinned_flag = nullptr; // host-pinned done, iter, done_iter; kept between renders
 	// Instantiated graphs of the last graph render, with the launch arguments
 	// and iteration counts they have baked in; reused while those stay the
@@ -619,6 +749,7 @@ void wavefront_release() {
 	ww.state.release();
 	ww.hit.release();
 	ww.shadow.release();
+	ww.acc.release();
 }
 
 void fill_zero(void *p, size_t bytes, hipStream_t s) {
@@ -708,9 +839,22 @@ void wavefront_render_on(const WaveItem *items, int count, const WaveConfig &cfg
 		w.count = (unsigned long long)w.npix_pad * (unsigned long long)it.params.spp;
 		total += w.count;
 		max_depth = std::max(max_depth, it.scene.depth);
-		fill_zero(it.out, (size_t)it.params.width * it.params.height * 3 * 4, s);
 	}
 	WaveWork &ww = work();
+	// Radiance slots (12 bytes per path of the work range) when the range
+	// has 32-bit indices and its slots take at most a third of the free
+	// device memory: no float atomics, and each pixel is the same sum as the
+	// megakernel's. Otherwise (or in caller memory, or when asked) the
+	// contributions go to the zeroed frame buffers with float atomics.
+	bool slots = !ext && !cfg.fb_atomics && total <= 0xFFFFFFFFull;
+	if (slots) {
+		size_t freeb = 0, totb = 0;
+		RN_HIP(hipMemGetInfo(&freeb, &totb));
+		slots = total * 12ull <= (freeb + ww.acc.bytes) / 3;
+	}
+	if (!slots)
+		for (int i = 0; i < count; i++)
+			fill_zero(items[i].out, (size_t)items[i].params.width * items[i].params.height * 3 * 4, s);
 	// Paths in flight. Each launch of the persistent trace kernels ends with
 	// a tail in which its waves drain, so fewer, larger launches win: at
 	// 1,048,588 triangles / 1080p / 64 spp a frame took 142 ms with 2M paths
@@ -762,6 +906,7 @@ void wavefront_render_on(const WaveItem *items, int count, const WaveConfig &cfg
 	q.sd = sh4 + cap;
 	q.sc = sh4 + 2 * (size_t)cap;
 	q.ctl = static_cast<WCtl *>(take(ww.ctl, sizeof(WCtl)));
+	q.acc = slots ? static_cast<float *>(ww.acc.reserve((size_t)total * 3 * sizeof(float))) : nullptr;
 	WScene *dsc = static_cast<WScene *>(take(ww.scenes, hs.size() * sizeof(WScene)));
 	if (ext && (size_t)(carve - static_cast<char *>(ext->base)) > ext->bytes)
 		throw Error("wavefront: internal error, caller work memory overrun"); // excluded by wave_scratch_capacity
@@ -944,9 +1089,17 @@ void wavefront_render_on(const WaveItem *items, int count, const WaveConfig &cfg
 		st.shadow_ms = acc[2];
 	}
 	for (int i = 0; i < count; i++) {
-		size_t n = (size_t)items[i].params.width * items[i].params.height * 3;
-		k_resolve<<<(unsigned)std::min<size_t>(2048, (n + 255) / 256), 256, 0, s>>>(items[i].out, n,
-		                                                                            1.0f / (float)items[i].params.spp);
+		const float scale = 1.0f / (float)items[i].params.spp;
+		if (slots) {
+			const WScene &w = hs[(size_t)i];
+			const unsigned np = (unsigned)w.npix_pad;
+			k_reduce_slots<<<(unsigned)std::min<size_t>(8192, (np + 255) / 256), 256, 0, s>>>(
+			    items[i].out, q.acc + 3 * (size_t)w.first, items[i].params.width, items[i].params.height, w.tiles_x,
+			    np, items[i].params.spp, scale);
+		} else {
+			size_t n = (size_t)items[i].params.width * items[i].params.height * 3;
+			k_resolve<<<(unsigned)std::min<size_t>(2048, (n + 255) / 256), 256, 0, s>>>(items[i].out, n, scale);
+		}
 	}
 	RN_HIP(hipGetLastError());
 }

@@ -375,6 +375,28 @@ def test_wavefront_matches_megakernel(rn, oracle):
     assert s_wf["rays_shadow"] == s_mk["rays_shadow"]
 
 
+@pytest.mark.parametrize("shape", [(96, 64, 16, 20000), (97, 61, 5, 6000)])
+def test_wavefront_slots_are_the_megakernel_sums(rn, oracle, shape):
+    """With radiance slots (the default) the wavefront adds a path's
+    contributions in depth order and a pixel's paths in sample order, as the
+    megakernel does: the same bits, on every run, for an image whose size is
+    not a multiple of the 8x8 tile and with a queue that refills many times.
+    With RN_FLAG_FB_ATOMICS (float atomics, arbitrary order) the image agrees
+    to float rounding."""
+    W, H, spp, queue = shape
+    sd, sc, cam, p, oc, op = cornell_setup(rn, oracle, W, H, spp, seed=3)
+    mk = sc.render_hdr(cam, p)
+    p.integrator = rn.INTEGRATOR_WAVEFRONT
+    p.queue_paths = queue
+    wf = sc.render_hdr(cam, p)
+    assert np.isfinite(wf).all() and wf.mean() > 0
+    assert np.array_equal(wf, mk)
+    assert np.array_equal(sc.render_hdr(cam, p), wf)
+    p.flags = rn.FLAG_FB_ATOMICS
+    at = sc.render_hdr(cam, p)
+    assert np.allclose(at, mk, rtol=1e-5, atol=1e-6)
+
+
 def test_mode_r_matches_oracle_same_stream(rn, oracle):
     """GPU reference estimator vs the CPU restatement with the same Philox
     stream: most 8-bit pixels identical (libm sin/cos/pow/exp and FMA differ)."""
