@@ -200,7 +200,8 @@ typedef struct {
 	uint64_t nodes_shadow, tris_shadow;
 	double extend_ms, shade_ms, shadow_ms; /* wavefront stage times (RN_FLAG_TIME_STAGES): extend_ms is the
 	                                        * trace launches (closest hits, then the previous iteration's any-hit
-	                                        * rays, in one launch), shade_ms generate + shade; shadow_ms is 0 */
+	                                        * rays, in one launch), shade_ms generate + shade + the one-lane
+	                                        * queue advance; shadow_ms is 0 */
 	uint64_t iterations;                  /* wavefront iterations */
 } rn_stats;
 

@@ -276,7 +276,9 @@ RN_DEV void slot_store(float *p, float3 v) {
 }
 
 // New camera paths for queue entries [cur_count, cur_count + gen_count).
-__global__ __launch_bounds__(256) void k_generate(WQ q, int cur) {
+// The scene table is a kernel argument of its own, marked restrict: nothing
+// the kernel stores aliases it, so loads at a uniform index are scalar loads.
+__global__ __launch_bounds__(256) void k_generate(WQ q, int cur, const WScene *__restrict__ scenes) {
 	WCtl *c = q.ctl;
 	const unsigned ncur = c->cur_count, n = ncur + c->gen_count;
 	const unsigned long long gbase = c->gen_base;
@@ -287,11 +289,17 @@ __global__ __launch_bounds__(256) void k_generate(WQ q, int cur) {
 		if (i >= n)
 			continue;
 		unsigned long long g = gbase + (i - ncur);
-		int si = find_scene(q.scenes, q.nscenes, g);
-		const WScene &ws = q.scenes[si];
+		const int si = q.nscenes == 1 ? 0 : find_scene(scenes, q.nscenes, g);
+		const WScene &ws = scenes[si];
 		unsigned long long local = g - ws.first;
-		uint32_t smp = ws.p.sample_base + (uint32_t)(local / (unsigned long long)ws.npix_pad);
-		uint32_t pp = (uint32_t)(local % (unsigned long long)ws.npix_pad);
+		uint32_t smp, pp;
+		if ((local >> 32) == 0) { // 32-bit division for work ranges below 2^32
+			smp = ws.p.sample_base + (uint32_t)local / (uint32_t)ws.npix_pad;
+			pp = (uint32_t)local % (uint32_t)ws.npix_pad;
+		} else {
+			smp = ws.p.sample_base + (uint32_t)(local / (unsigned long long)ws.npix_pad);
+			pp = (uint32_t)(local % (unsigned long long)ws.npix_pad);
+		}
 		uint32_t tile = pp >> 6, t = pp & 63u;
 		int x = (int)(tile % (uint32_t)ws.tiles_x) * kPixTile + (int)(t & 7u);
 		int y = (int)(tile / (uint32_t)ws.tiles_x) * kPixTile + (int)(t >> 3);
@@ -526,11 +534,11 @@ constexpr int kShadeWaves = kShadeBlock / 64;
 // for. The batch instance, at 74 VGPRs, runs one block per CU either way.
 template <bool ONE>
 __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(ONE ? 8 : 0, ONE ? 8 : 0))) void k_shade(
-    WQ q, int cur) {
+    WQ q, int cur, const WScene *__restrict__ scenes) {
 	if (instance_skips<ONE>(q))
 		return;
 	const unsigned one = ONE ? (unsigned)q.ctl->single : 0u;
-	const SlotMap m1 = ONE && q.acc ? slot_map_uniform(q.scenes[one]) : SlotMap{};
+	const SlotMap m1 = ONE && q.acc ? slot_map_uniform(scenes[one]) : SlotMap{};
 	static_assert(kShadeWaves <= 64 && kShadeBlock < 65536, "block counts are packed in 16-bit halves");
 	__shared__ unsigned wcnt[kShadeWaves]; // per wave: shadow pushes | continuing pushes << 16
 	__shared__ unsigned bbase[2];
@@ -565,7 +573,7 @@ __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(ONE
 				d4 = get4(D + i);
 				beta = xyz(b4);
 				// one scene: a uniform address, loaded once into scalar registers
-				const WScene &ws = q.scenes[ONE ? one : (meta & 0xFFFFu)];
+				const WScene &ws = scenes[ONE ? one : (meta & 0xFFFFu)];
 				const DevScene &sc = ws.sc;
 				const DevParams &p = ws.p;
 				float3 o = xyz(o4), d = xyz(d4);
@@ -929,16 +937,18 @@ void wavefront_render_on(const WaveItem *items, int count, const WaveConfig &cfg
 	RN_HIP(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
 	const bool deep = max_depth - 1 > kLdsDepth;
 
-	// Stage timing: events bracket each launch; pairs are summed at the end.
+	// Stage timing: one event at each boundary between stages (3 per
+	// iteration; every event costs GPU time); interval k, from event k to
+	// event k + 1, belongs to stage stage_of[k] and the intervals are summed
+	// at the end. The advance kernel counts with the shade stage.
 	const bool timed = cfg.time_stages && !cfg.use_graph;
 	std::vector<int> stage_of;
 	size_t nev = 0;
-	auto mark = [&](int stage, bool begin) {
+	auto mark = [&](int stage_before) {
 		if (!timed)
 			return;
 		RN_HIP(hipEventRecord(pool_event(ww, nev++), s));
-		if (begin)
-			stage_of.push_back(stage);
+		stage_of.push_back(stage_before);
 	};
 	const int bs = cfg.block == 128 ? 128 : 256;
 	// Trace kernels: enough resident waves to fill every SIMD; each wave
@@ -951,10 +961,10 @@ void wavefront_render_on(const WaveItem *items, int count, const WaveConfig &cfg
 	const unsigned shgrid = std::min<unsigned>((unsigned)std::max(1, n_cu) * (2048u / kShadeBlock),
 	                                           (cap + kShadeBlock - 1) / kShadeBlock);
 	auto iteration = [&](int cur) {
-		mark(1, true);
-		k_generate<<<sgrid, 256, 0, s>>>(q, cur);
-		mark(1, false);
-		mark(0, true);
+		if (timed && nev == 0)
+			RN_HIP(hipEventRecord(pool_event(ww, nev++), s));
+		k_generate<<<sgrid, 256, 0, s>>>(q, cur, q.scenes);
+		mark(1);
 #define RN_LAUNCH1(K, LDSD, SPD, DEEP, ONE, ...)                                                                      \
 	do {                                                                                                               \
 		if (DEEP && bs == 128)                                                                                         \
@@ -973,15 +983,14 @@ void wavefront_render_on(const WaveItem *items, int count, const WaveConfig &cfg
 			RN_LAUNCH1(K, LDSD, SPD, DEEP, false, __VA_ARGS__);                                                        \
 	} while (0)
 		RN_LAUNCH(k_trace, kLdsDepth, kSpillDepth, deep, q, cur);
-		mark(0, false);
-		mark(1, true);
-		k_shade<true><<<shgrid, kShadeBlock, 0, s>>>(q, cur);
+		mark(0);
+		k_shade<true><<<shgrid, kShadeBlock, 0, s>>>(q, cur, q.scenes);
 		if (count > 1)
-			k_shade<false><<<shgrid, kShadeBlock, 0, s>>>(q, cur);
-		mark(1, false);
+			k_shade<false><<<shgrid, kShadeBlock, 0, s>>>(q, cur, q.scenes);
 #undef RN_LAUNCH
 #undef RN_LAUNCH1
 		k_advance<<<1, 1, 0, s>>>(q, 0);
+		mark(1);
 	};
 
 
@@ -1081,7 +1090,7 @@ void wavefront_render_on(const WaveItem *items, int count, const WaveConfig &cfg
 		double acc[3] = {0, 0, 0};
 		for (size_t k = 0; k < stage_of.size(); k++) {
 			float ms = 0;
-			RN_HIP(hipEventElapsedTime(&ms, ww.events[2 * k], ww.events[2 * k + 1]));
+			RN_HIP(hipEventElapsedTime(&ms, ww.events[k], ww.events[k + 1]));
 			acc[stage_of[k]] += ms;
 		}
 		st.extend_ms = acc[0];
