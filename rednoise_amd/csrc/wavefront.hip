@@ -1074,6 +1074,18 @@ void wavefront_render_on(const WaveItem *items, int count, const WaveConfig &cfg
 		}
 		iters += n;
 		n = (uint64_t)per;
+		// The images from the slots, enqueued before the done flag is read:
+		// the reduce only reads the slots and overwrites the images, so when
+		// more iterations turn out to be needed it simply runs again after
+		// them (no host round trip between the last iteration and the reduce).
+		if (slots)
+			for (int i = 0; i < count; i++) {
+				const WScene &w = hs[(size_t)i];
+				const unsigned np = (unsigned)w.npix_pad;
+				k_reduce_slots<<<(unsigned)std::min<size_t>(8192, (np + 255) / 256), 256, 0, s>>>(
+				    items[i].out, q.acc + 3 * (size_t)w.first, items[i].params.width, items[i].params.height,
+				    w.tiles_x, np, items[i].params.spp, 1.0f / (float)items[i].params.spp);
+			}
 		RN_HIP(hipGetLastError());
 		RN_HIP(hipMemcpyAsync(flag, &q.ctl->done, 3 * sizeof(unsigned), hipMemcpyDeviceToHost, s));
 		RN_HIP(hipStreamSynchronize(s));
@@ -1097,19 +1109,12 @@ void wavefront_render_on(const WaveItem *items, int count, const WaveConfig &cfg
 		st.shade_ms = acc[1];
 		st.shadow_ms = acc[2];
 	}
-	for (int i = 0; i < count; i++) {
-		const float scale = 1.0f / (float)items[i].params.spp;
-		if (slots) {
-			const WScene &w = hs[(size_t)i];
-			const unsigned np = (unsigned)w.npix_pad;
-			k_reduce_slots<<<(unsigned)std::min<size_t>(8192, (np + 255) / 256), 256, 0, s>>>(
-			    items[i].out, q.acc + 3 * (size_t)w.first, items[i].params.width, items[i].params.height, w.tiles_x,
-			    np, items[i].params.spp, scale);
-		} else {
+	if (!slots)
+		for (int i = 0; i < count; i++) {
 			size_t n = (size_t)items[i].params.width * items[i].params.height * 3;
-			k_resolve<<<(unsigned)std::min<size_t>(2048, (n + 255) / 256), 256, 0, s>>>(items[i].out, n, scale);
+			k_resolve<<<(unsigned)std::min<size_t>(2048, (n + 255) / 256), 256, 0, s>>>(
+			    items[i].out, n, 1.0f / (float)items[i].params.spp);
 		}
-	}
 	RN_HIP(hipGetLastError());
 }
 
