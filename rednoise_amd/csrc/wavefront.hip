@@ -229,6 +229,15 @@ RN_DEV unsigned lanes_below(uint64_t m) {
 // meta word: scene index (low 16 bits) | depth << 16; depth 255 = dead slot.
 constexpr uint32_t kDead = 255u;
 
+// Compact camera paths. In a one-scene render a camera path's queue entry
+// holds only its direction record (d.xyz, sample): its origin is the camera,
+// its throughput 1, its meta word (scene 0, depth 0), and its pixel follows
+// from the work index gen_base + (i - cur_count) of entry i >= cur_count (the
+// camera paths k_generate appended). A sample of kDeadSample marks an entry
+// outside the image (tile padding). Batches keep full records.
+constexpr uint32_t kDeadSample = 0xFFFFFFFFu;
+RN_DEV bool compact_camera_paths(const WQ &q) { return q.nscenes == 1; }
+
 // Radiance slot of the path of pixel `pix`, sample `smp`: its global work
 // index, i.e. the inverse of k_generate's order (sample-major, 8x8 tiles,
 // rows within a tile). A camera path's slot is the work index k_generate
@@ -249,6 +258,15 @@ RN_DEV SlotMap slot_map_uniform(const WScene &ws) {
 	return SlotMap{(uint32_t)__builtin_amdgcn_readfirstlane(m.first), (uint32_t)__builtin_amdgcn_readfirstlane(m.width),
 	               (uint32_t)__builtin_amdgcn_readfirstlane(m.tiles_x),
 	               (uint32_t)__builtin_amdgcn_readfirstlane(m.npix_pad), (uint32_t)__builtin_amdgcn_readfirstlane(m.sbase)};
+}
+// The pixel of the camera path of work index g and sample smp (the inverse
+// of k_generate's tiling: smp is the sample plane, the remainder the tiled
+// pixel index).
+RN_DEV uint32_t camera_pixel(const SlotMap &m, unsigned long long g, uint32_t smp) {
+	const uint32_t pp = (uint32_t)(g - m.first - (unsigned long long)(smp - m.sbase) * m.npix_pad);
+	const uint32_t tile = pp >> 6, t = pp & 63u;
+	const uint32_t x = (tile % m.tiles_x) * kPixTile + (t & 7u), y = (tile / m.tiles_x) * kPixTile + (t >> 3);
+	return y * m.width + x;
 }
 RN_DEV uint32_t path_slot(const SlotMap &m, uint32_t pix, uint32_t smp) {
 	const uint32_t y = pix / m.width, x = pix - y * m.width;
@@ -314,9 +332,15 @@ __global__ __launch_bounds__(256) void k_generate(WQ q, int cur, const WScene *_
 			meta = (uint32_t)si;
 			npaths++;
 		}
-		put4(O + i, o4);
-		put4(D + i, d4);
-		put4(B + i, make_float4(1.f, 1.f, 1.f, __uint_as_float(meta)));
+		if (compact_camera_paths(q)) {
+			if ((meta >> 16) == kDead)
+				d4.w = __uint_as_float(kDeadSample);
+			put4(D + i, d4);
+		} else {
+			put4(O + i, o4);
+			put4(D + i, d4);
+			put4(B + i, make_float4(1.f, 1.f, 1.f, __uint_as_float(meta)));
+		}
 	}
 	wave_add_u64(&q.cnt->shard[blockIdx.x % kCounterShards].paths, npaths);
 }
@@ -383,6 +407,12 @@ RN_DEV void trace_queue(const WQ &q, int cur, uint64_t *stk, const uint8_t *octa
 	const Bvh8Node *nodes = ONE ? q.scenes[one].sc.nodes : nullptr;
 	const float4 *tris = ONE ? q.scenes[one].sc.tris : nullptr;
 	float *const out_one = ONE ? q.scenes[one].out : nullptr;
+	// compact camera paths (entries from cur_count on) start at the eye
+	const bool compact = compact_camera_paths(q);
+	const unsigned ncur = c->cur_count;
+	const float4 eye =
+	    ONE ? make_float4(q.scenes[one].cam.eye[0], q.scenes[one].cam.eye[1], q.scenes[one].cam.eye[2], 0.f)
+	        : make_float4(0.f, 0.f, 0.f, 0.f);
 	Trav<SPILL> t;
 	unsigned nrays = 0; // this lane's rays (at most the queue length, below 2^31)
 	while (true) {
@@ -422,10 +452,17 @@ RN_DEV void trace_queue(const WQ &q, int cur, uint64_t *stk, const uint8_t *octa
 					// the entry's loads are issued together
 					// path records stream past non-temporally; shadow records
 					// load plainly (0.26 ms per frame faster than streamed)
-					const float4 o4 = QUERY == 0 ? get4(O + my) : O[my], d4 = QUERY == 0 ? get4(D + my) : D[my];
+					// a compact camera path: its direction record only
+					const bool cam = QUERY == 0 && ONE && compact && my >= ncur;
+					const float4 d4 = QUERY == 0 ? get4(D + my) : D[my];
+					const float4 o4 = cam ? eye : (QUERY == 0 ? get4(O + my) : O[my]);
 					// closest hit: the path's meta word; any hit: the
 					// contribution and scene index
-					const float4 m4 = QUERY == 1 ? M[my] : make_float4(0.f, 0.f, 0.f, M[my].w);
+					const float4 m4 = QUERY == 1 ? M[my]
+					                 : make_float4(0.f, 0.f, 0.f,
+					                               cam ? __uint_as_float(__float_as_uint(d4.w) == kDeadSample
+					                                                         ? kDead << 16 : 0u)
+					                                   : M[my].w);
 					const uint32_t meta = __float_as_uint(m4.w);
 					if (QUERY == 1 || (meta >> 16) != kDead) {
 						if (!ONE) {
@@ -539,6 +576,18 @@ __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(ONE
 		return;
 	const unsigned one = ONE ? (unsigned)q.ctl->single : 0u;
 	const SlotMap m1 = ONE && q.acc ? slot_map_uniform(scenes[one]) : SlotMap{};
+	const bool compact = compact_camera_paths(q);
+	// the camera and the map of a compact camera path's work index, kept in
+	// LDS (scalar registers are at their limit for two blocks per CU)
+	__shared__ float cam_eye[3];
+	__shared__ SlotMap cam_map;
+	if (ONE && compact) {
+		if (threadIdx.x < 3)
+			cam_eye[threadIdx.x] = scenes[one].cam.eye[threadIdx.x];
+		if (threadIdx.x == 0)
+			cam_map = slot_map(scenes[one]);
+		__syncthreads();
+	}
 	static_assert(kShadeWaves <= 64 && kShadeBlock < 65536, "block counts are packed in 16-bit halves");
 	__shared__ unsigned wcnt[kShadeWaves]; // per wave: shadow pushes | continuing pushes << 16
 	__shared__ unsigned bbase[2];
@@ -555,7 +604,13 @@ __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(ONE
 		const unsigned i = base + threadIdx.x;
 		uint32_t meta = kDead << 16;
 		float4 b4 = make_float4(0.f, 0.f, 0.f, 0.f);
-		if (i < n) {
+		// a compact camera path: its direction record only (scene 0, depth 0,
+		// throughput 1)
+		const bool cam = ONE && compact && i >= ncur && i < n;
+		if (cam) {
+			meta = __float_as_uint(D[i].w) == kDeadSample ? kDead << 16 : 0u;
+			b4 = make_float4(1.f, 1.f, 1.f, __uint_as_float(meta));
+		} else if (i < n) {
 			b4 = get4(B + i);
 			meta = __float_as_uint(b4.w);
 		}
@@ -569,8 +624,10 @@ __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(ONE
 			const int prim = geti(q.hitp + i);
 			if (prim >= 0) {
 				const float4 hn = get4(q.hitn + i);
-				o4 = get4(O + i);
 				d4 = get4(D + i);
+				o4 = cam ? make_float4(cam_eye[0], cam_eye[1], cam_eye[2],
+				                       __uint_as_float(camera_pixel(cam_map, gbase + (i - ncur), __float_as_uint(d4.w))))
+				         : get4(O + i);
 				beta = xyz(b4);
 				// one scene: a uniform address, loaded once into scalar registers
 				const WScene &ws = scenes[ONE ? one : (meta & 0xFFFFu)];
@@ -830,6 +887,10 @@ void wavefront_render_on(const WaveItem *items, int count, const WaveConfig &cfg
                          WaveStats &st, const WaveScratch *ext) {
 	if (count <= 0 || count > 65535)
 		throw Error("wavefront: batch size must be in [1, 65535]");
+	// a one-scene render marks padding entries with the sample kDeadSample
+	if (count == 1 && (unsigned long long)items[0].params.sample_base + (unsigned long long)items[0].params.spp >=
+	                      (unsigned long long)kDeadSample)
+		throw Error("wavefront: sample_base + spp must stay below 2^32 - 1");
 	std::vector<WScene> hs((size_t)count);
 	unsigned long long total = 0;
 	int max_depth = 0;
