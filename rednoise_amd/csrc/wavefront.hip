@@ -229,6 +229,44 @@ RN_DEV unsigned lanes_below(uint64_t m) {
 // meta word: scene index (low 16 bits) | depth << 16; depth 255 = dead slot.
 constexpr uint32_t kDead = 255u;
 
+// The scene table seen through the constant address space: the kernels
+// never write it, and a load through it at a uniform index is a scalar load
+// wherever it stands (through a generic pointer the compiler makes a load
+// that follows a store in the same loop a vector load).
+typedef __attribute__((address_space(4))) const WScene cWScene;
+
+// Field-wise copies out of the constant-address-space table (a struct copy
+// would need a copy constructor taking the qualified type).
+RN_DEV DevScene scene_of(const cWScene &w) {
+	DevScene d;
+	d.nodes = w.sc.nodes;
+	d.tris = w.sc.tris;
+	d.albedo = w.sc.albedo;
+	d.emission = w.sc.emission;
+	d.light_tri = w.sc.light_tri;
+	d.light_cdf = w.sc.light_cdf;
+	d.light_geom = w.sc.light_geom;
+	d.n = w.sc.n;
+	d.nl = w.sc.nl;
+	d.total_light_area = w.sc.total_light_area;
+	d.depth = w.sc.depth;
+	return d;
+}
+RN_DEV DevParams params_of(const cWScene &w) {
+	DevParams d;
+	d.width = w.p.width;
+	d.height = w.p.height;
+	d.spp = w.p.spp;
+	d.max_depth = w.p.max_depth;
+	d.rr_depth = w.p.rr_depth;
+	d.rr_prob = w.p.rr_prob;
+	d.seed = w.p.seed;
+	d.scene_id = w.p.scene_id;
+	d.ray_eps = w.p.ray_eps;
+	d.sample_base = w.p.sample_base;
+	return d;
+}
+
 // Compact camera paths. In a one-scene render a camera path's queue entry
 // holds only its direction record (d.xyz, sample): its origin is the camera,
 // its throughput 1, its meta word (scene 0, depth 0), and its pixel follows
@@ -615,119 +653,130 @@ __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(ONE
 			meta = __float_as_uint(b4.w);
 		}
 		const uint32_t depth = meta >> 16;
-		bool push_sh = false, push_next = false;
-		float3 so = f3(0.f, 0.f, 0.f), wo = so, beta = so, contrib = so, shw = so;
-		float shmax = 0.f;
-		float4 o4 = make_float4(0.f, 0.f, 0.f, 0.f), d4 = o4;
-		uint32_t aux = 0; // slot mode: the path's radiance slot
-		if (depth != kDead) {
-			const int prim = geti(q.hitp + i);
-			if (prim >= 0) {
-				const float4 hn = get4(q.hitn + i);
-				d4 = get4(D + i);
-				o4 = cam ? make_float4(cam_eye[0], cam_eye[1], cam_eye[2],
-				                       __uint_as_float(camera_pixel(cam_map, gbase + (i - ncur), __float_as_uint(d4.w))))
-				         : get4(O + i);
-				beta = xyz(b4);
-				// one scene: a uniform address, loaded once into scalar registers
-				const WScene &ws = scenes[ONE ? one : (meta & 0xFFFFu)];
-				const DevScene &sc = ws.sc;
-				const DevParams &p = ws.p;
-				float3 o = xyz(o4), d = xyz(d4);
-				uint32_t pix = __float_as_uint(o4.w), smp = __float_as_uint(d4.w);
-				float3 P = o + d * hn.w;
-				float3 ng = facing_from_cross(xyz(hn), d);
-				// slot mode: the path's slot and its value once this vertex's
-				// emission is in (the any-hit ray pushed below carries that
-				// value plus its contribution, stored if it is unoccluded)
-				float3 sv = f3(0.f, 0.f, 0.f);
-				if (depth == 0) {
-					float3 le = load3(sc.emission, prim);
-					if (q.acc) {
-						// the slot's first value: 0 + beta * le (the
-						// megakernel's first L + beta * le)
-						aux = (uint32_t)(gbase + (i - ncur));
-						sv = f3(beta.x * le.x, beta.y * le.y, beta.z * le.z);
-						slot_store(q.acc + 3 * (size_t)aux, sv);
-					} else if (le.x != 0.f || le.y != 0.f || le.z != 0.f) {
-						float *fb = ws.out + 3 * (size_t)pix;
-						atomic_add_global(fb, beta.x * le.x);
-						atomic_add_global(fb + 1, beta.y * le.y);
-						atomic_add_global(fb + 2, beta.z * le.z);
+		{
+			bool push_sh = false, push_next = false;
+			float3 so = f3(0.f, 0.f, 0.f), wo = so, beta = so, contrib = so, shw = so;
+			float shmax = 0.f;
+			float4 o4 = make_float4(0.f, 0.f, 0.f, 0.f), d4 = o4;
+			uint32_t aux = 0; // slot mode: the path's radiance slot
+			auto shade_vertex = [&](const cWScene &ws) {
+				const int prim = geti(q.hitp + i);
+				if (prim >= 0) {
+					const float4 hn = get4(q.hitn + i);
+					d4 = get4(D + i);
+					o4 = cam ? make_float4(cam_eye[0], cam_eye[1], cam_eye[2],
+					                       __uint_as_float(camera_pixel(cam_map, gbase + (i - ncur), __float_as_uint(d4.w))))
+					         : get4(O + i);
+					beta = xyz(b4);
+					// copies through the constant address space: scalar loads
+					// at the round's (uniform) scene index
+					const DevScene sc = scene_of(ws);
+					const DevParams p = params_of(ws);
+					float3 o = xyz(o4), d = xyz(d4);
+					uint32_t pix = __float_as_uint(o4.w), smp = __float_as_uint(d4.w);
+					float3 P = o + d * hn.w;
+					float3 ng = facing_from_cross(xyz(hn), d);
+					// slot mode: the path's slot and its value once this vertex's
+					// emission is in (the any-hit ray pushed below carries that
+					// value plus its contribution, stored if it is unoccluded)
+					float3 sv = f3(0.f, 0.f, 0.f);
+					if (depth == 0) {
+						float3 le = load3(sc.emission, prim);
+						if (q.acc) {
+							// the slot's first value: 0 + beta * le (the
+							// megakernel's first L + beta * le)
+							aux = (uint32_t)(gbase + (i - ncur));
+							sv = f3(beta.x * le.x, beta.y * le.y, beta.z * le.z);
+							slot_store(q.acc + 3 * (size_t)aux, sv);
+						} else if (le.x != 0.f || le.y != 0.f || le.z != 0.f) {
+							float *fb = (float *)ws.out + 3 * (size_t)pix;
+							atomic_add_global(fb, beta.x * le.x);
+							atomic_add_global(fb + 1, beta.y * le.y);
+							atomic_add_global(fb + 2, beta.z * le.z);
+						}
+					} else if (q.acc) {
+						aux = path_slot(ONE ? m1
+						                        : SlotMap{(uint32_t)ws.first, (uint32_t)ws.p.width, (uint32_t)ws.tiles_x,
+						                                  (uint32_t)ws.npix_pad, ws.p.sample_base},
+						                pix, smp);
 					}
-				} else if (q.acc) {
-					aux = path_slot(ONE ? m1 : slot_map(ws), pix, smp);
-				}
-				float3 a = load3(sc.albedo, prim);
-				so = P + ng * p.ray_eps;
-				u32x4 rn = philox4x32_10(pix, smp, depth, 1u, p.seed, p.scene_id);
-				if (sc.nl > 0) {
-					NeeSample ns;
-					if (sample_light(sc, so, ng, u01(rn.x), u01(rn.y), u01(rn.z), ns)) {
-						float3 le = load3(sc.emission, ns.light);
-						contrib = beta * a * (le * ns.Y);
-						if (q.acc) // the slot's value: this vertex's emission at depth 0, else read here
-							contrib = add_uncontracted(depth == 0 ? sv : slot_load(q.acc + 3 * (size_t)aux), contrib);
-						shw = ns.wi;
-						shmax = ns.tmax;
-						push_sh = true;
+					float3 a = load3(sc.albedo, prim);
+					so = P + ng * p.ray_eps;
+					u32x4 rn = philox4x32_10(pix, smp, depth, 1u, p.seed, p.scene_id);
+					if (sc.nl > 0) {
+						NeeSample ns;
+						if (sample_light(sc, so, ng, u01(rn.x), u01(rn.y), u01(rn.z), ns)) {
+							float3 le = load3(sc.emission, ns.light);
+							contrib = beta * a * (le * ns.Y);
+							if (q.acc) // the slot's value: this vertex's emission at depth 0, else read here
+								contrib = add_uncontracted(depth == 0 ? sv : slot_load(q.acc + 3 * (size_t)aux), contrib);
+							shw = ns.wi;
+							shmax = ns.tmax;
+							push_sh = true;
+						}
 					}
-				}
-				if ((int)depth + 1 < p.max_depth) {
-					u32x4 rb = philox4x32_10(pix, smp, depth, 2u, p.seed, p.scene_id);
-					wo = cosine_dir(ng, u01(rb.x), u01(rb.y));
-					beta = beta * a;
-					push_next = true;
-					if ((int)depth >= p.rr_depth) {
-						float qs = rr_survival(p, beta);
-						if (!(u01(rn.w) < qs))
-							push_next = false;
-						else
-							beta = beta * (1.0f / qs);
+					if ((int)depth + 1 < p.max_depth) {
+						u32x4 rb = philox4x32_10(pix, smp, depth, 2u, p.seed, p.scene_id);
+						wo = cosine_dir(ng, u01(rb.x), u01(rb.y));
+						beta = beta * a;
+						push_next = true;
+						if ((int)depth >= p.rr_depth) {
+							float qs = rr_survival(p, beta);
+							if (!(u01(rn.w) < qs))
+								push_next = false;
+							else
+								beta = beta * (1.0f / qs);
+						}
 					}
+				} else if (depth == 0 && q.acc) {
+					// a camera ray that left the scene: its slot holds zero
+					slot_store(q.acc + 3 * (size_t)(uint32_t)(gbase + (i - ncur)), f3(0.f, 0.f, 0.f));
 				}
-			} else if (depth == 0 && q.acc) {
-				// a camera ray that left the scene: its slot holds zero
-				slot_store(q.acc + 3 * (size_t)(uint32_t)(gbase + (i - ncur)), f3(0.f, 0.f, 0.f));
+					};
+			// the single-scene instance reads its scene's fields with scalar
+			// loads; the batch instance reads each lane's scene (a loop over
+			// the scenes of a wave, to shade each with scalar loads, was 8%
+			// slower on the 64-scene batch: one more barrier per block)
+			if (depth != kDead)
+				shade_vertex(((const cWScene *)scenes)[ONE ? one : (meta & 0xFFFFu)]);
+			// Block-wide compaction: each wave's two push counts packed in one
+			// word (both at most 64, block sums below 2^16); thread 0 sums them
+			// and takes the block's entries with one 64-bit atomic; each wave adds
+			// the words of the waves before it.
+			const uint64_t msh = __ballot(push_sh), mnx = __ballot(push_next);
+			if (lane == 0)
+				wcnt[wave] = (unsigned)__popcll(msh) | ((unsigned)__popcll(mnx) << 16);
+			__syncthreads();
+			if (threadIdx.x == 0) {
+				unsigned tot = 0;
+				for (int v = 0; v < kShadeWaves; v++)
+					tot += wcnt[v];
+				unsigned long long old = 0;
+				if (tot)
+					old = atomicAdd(reinterpret_cast<unsigned long long *>(&c->next_count),
+					                (unsigned long long)(tot >> 16) | ((unsigned long long)(tot & 0xFFFFu) << 32));
+				bbase[0] = (unsigned)(old >> 32);
+				bbase[1] = (unsigned)old;
 			}
-		}
-		// Block-wide compaction: each wave's two push counts packed in one
-		// word (both at most 64, block sums below 2^16); thread 0 sums them
-		// and takes the block's entries with one 64-bit atomic; each wave adds
-		// the words of the waves before it.
-		const uint64_t msh = __ballot(push_sh), mnx = __ballot(push_next);
-		if (lane == 0)
-			wcnt[wave] = (unsigned)__popcll(msh) | ((unsigned)__popcll(mnx) << 16);
-		__syncthreads();
-		if (threadIdx.x == 0) {
-			unsigned tot = 0;
-			for (int v = 0; v < kShadeWaves; v++)
-				tot += wcnt[v];
-			unsigned long long old = 0;
-			if (tot)
-				old = atomicAdd(reinterpret_cast<unsigned long long *>(&c->next_count),
-				                (unsigned long long)(tot >> 16) | ((unsigned long long)(tot & 0xFFFFu) << 32));
-			bbase[0] = (unsigned)(old >> 32);
-			bbase[1] = (unsigned)old;
-		}
-		__syncthreads();
-		unsigned before = 0;
-		for (int v = 0; v < wave; v++)
-			before += wcnt[v];
-		const unsigned ss = bbase[0] + (before & 0xFFFFu) + (unsigned)__popcll(msh & lt);
-		const unsigned ns = bbase[1] + (before >> 16) + (unsigned)__popcll(mnx & lt);
-		__syncthreads(); // wcnt and bbase are rewritten by the next iteration
-		// the shadow record's .w: the path's radiance slot, or its pixel
-		// without slots
-		if (push_sh) {
-			put4(q.so + ss, make_float4(so.x, so.y, so.z, q.acc ? __uint_as_float(aux) : o4.w));
-			put4(q.sd + ss, make_float4(shw.x, shw.y, shw.z, shmax));
-			put4(q.sc + ss, make_float4(contrib.x, contrib.y, contrib.z, __uint_as_float(meta & 0xFFFFu)));
-		}
-		if (push_next) {
-			put4(NO + ns, make_float4(so.x, so.y, so.z, o4.w));
-			put4(ND + ns, make_float4(wo.x, wo.y, wo.z, d4.w));
-			put4(NB + ns, make_float4(beta.x, beta.y, beta.z, __uint_as_float((meta & 0xFFFFu) | ((depth + 1) << 16))));
+			__syncthreads();
+			unsigned before = 0;
+			for (int v = 0; v < wave; v++)
+				before += wcnt[v];
+			const unsigned ss = bbase[0] + (before & 0xFFFFu) + (unsigned)__popcll(msh & lt);
+			const unsigned ns = bbase[1] + (before >> 16) + (unsigned)__popcll(mnx & lt);
+			__syncthreads(); // wcnt and bbase are rewritten by the next iteration
+			// the shadow record's .w: the path's radiance slot, or its pixel
+			// without slots
+			if (push_sh) {
+				put4(q.so + ss, make_float4(so.x, so.y, so.z, q.acc ? __uint_as_float(aux) : o4.w));
+				put4(q.sd + ss, make_float4(shw.x, shw.y, shw.z, shmax));
+				put4(q.sc + ss, make_float4(contrib.x, contrib.y, contrib.z, __uint_as_float(meta & 0xFFFFu)));
+			}
+			if (push_next) {
+				put4(NO + ns, make_float4(so.x, so.y, so.z, o4.w));
+				put4(ND + ns, make_float4(wo.x, wo.y, wo.z, d4.w));
+				put4(NB + ns, make_float4(beta.x, beta.y, beta.z, __uint_as_float((meta & 0xFFFFu) | ((depth + 1) << 16))));
+			}
 		}
 	}
 }
