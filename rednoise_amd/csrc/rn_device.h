@@ -328,6 +328,13 @@ RN_DEV float4 gload_f4(const float4 *base, uint32_t i) {
 	return make_float4(v.x, v.y, v.z, v.w);
 }
 
+// The x, y, z words of float4 record i (one three-word load).
+RN_DEV float3 gload_f3(const float4 *base, uint32_t i) {
+	typedef float nf3 __attribute__((ext_vector_type(3)));
+	const nf3 v = *(__attribute__((address_space(1))) const nf3 *)((const float *)base + 4 * (size_t)i);
+	return f3(v.x, v.y, v.z);
+}
+
 RN_DEV float gload_f(const float *p) { return *(__attribute__((address_space(1))) const float *)p; }
 RN_DEV int gload_i(const int *p) { return *(__attribute__((address_space(1))) const int *)p; }
 
@@ -468,7 +475,11 @@ RN_DEV bool trav_step(Trav<SPILL> &s, const Bvh8Node *__restrict__ nodes, const 
 			// The id test does not branch before the triangle test: a branch
 			// there lets the compiler sink the second and third loads behind
 			// the first, which costs a second memory round trip per test.
-			const float4 a = gload_f4(tris, 3 * ti), b4 = gload_f4(tris, 3 * ti + 1), c4 = gload_f4(tris, 3 * ti + 2);
+			// v1 and v2 load three words each: with a fourth, unused word the
+			// register allocator may reuse its register while the load is in
+			// flight and wait for it before issuing the next load
+			const float4 a = gload_f4(tris, 3 * ti);
+			const float3 b4 = gload_f3(tris, 3 * ti + 1), c4 = gload_f3(tris, 3 * ti + 2);
 			const int prim = __float_as_int(a.w);
 			const bool skip = prim == ignore;
 			if (COUNT && !skip)
@@ -476,9 +487,9 @@ RN_DEV bool trav_step(Trav<SPILL> &s, const Bvh8Node *__restrict__ nodes, const 
 			float tt, uu, vv;
 			bool ok;
 			if (TRI == 0)
-				ok = tri_watertight(s.wr, s.r.o, xyz(a), xyz(b4), xyz(c4), s.h.t, tt, uu, vv);
+				ok = tri_watertight(s.wr, s.r.o, xyz(a), b4, c4, s.h.t, tt, uu, vv);
 			else
-				ok = tri_compat(s.r.o, s.r.d, xyz(a), xyz(b4), xyz(c4), tt, uu, vv) && tt < s.h.t;
+				ok = tri_compat(s.r.o, s.r.d, xyz(a), b4, c4, tt, uu, vv) && tt < s.h.t;
 			if (ok && !skip) {
 				if (QUERY == 1) {
 					s.h.prim = 1;
@@ -490,7 +501,7 @@ RN_DEV bool trav_step(Trav<SPILL> &s, const Bvh8Node *__restrict__ nodes, const 
 				s.h.u = uu;
 				s.h.v = vv;
 				if (NG) // the vertices are in registers now; the shader needs no reload
-					s.h.ng = cross3(xyz(b4) - xyz(a), xyz(c4) - xyz(a));
+					s.h.ng = cross3(b4 - xyz(a), c4 - xyz(a));
 			}
 		}
 		s.thits = m;
