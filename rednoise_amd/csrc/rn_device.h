@@ -402,7 +402,13 @@ RN_DEV void trav_init(Trav<SPILL> &s, float3 o, float3 d, float tmax) {
 	s.h.u = 0.0f;
 	s.h.v = 0.0f;
 	s.sp = 0;
-	s.grp = mk_group(0u, 1u, 1u << (0u ^ s.r.oct));
+	// A ray with a NaN or infinite origin or direction component starts with
+	// nothing to visit: it misses (any hit: unoccluded), as it does against
+	// every triangle. (Left to the slab test, NaN slab distances vanish in
+	// the min/max and such a ray would enter every box of the tree.)
+	const bool finite = fabsf(o.x) < INFINITY && fabsf(o.y) < INFINITY && fabsf(o.z) < INFINITY &&
+	                    fabsf(d.x) < INFINITY && fabsf(d.y) < INFINITY && fabsf(d.z) < INFINITY;
+	s.grp = finite ? mk_group(0u, 1u, 1u << (0u ^ s.r.oct)) : 0ull;
 	s.tri_base = 0;
 	s.thits = 0;
 	s.tri_base2 = 0;

@@ -181,6 +181,25 @@ def test_flat_child_on_grid_line(rn):
     assert prim[0] == 0 and abs(t[0] - 0.1) < 1e-6
 
 
+def test_non_finite_rays_miss(rn):
+    """A ray with a NaN or infinite component misses every triangle (and is
+    unoccluded), as its comparisons against every triangle fail; the
+    traversal answers it at once instead of entering every box (NaN slab
+    distances vanish in the min/max)."""
+    v = soup(3000, seed=5)
+    sc = rn.Scene.from_arrays(v, np.full((len(v), 3), 0.5, np.float32))
+    nan, inf = np.float32(np.nan), np.float32(np.inf)
+    o = np.array([[0, 0, 0], [nan, 0, 0], [0, inf, 0], [0, 0, 0], [0, 0, 0], [0, 0, -inf]], np.float32)
+    d = np.array([[0, 0, 1], [0, 0, 1], [0, 0, 1], [nan, 0, 1], [0, -inf, 0], [0, 0, 1]], np.float32)
+    prim, t, u, w = sc.trace_rays(o, d)
+    assert (prim[1:] == -1).all()
+    occ = sc.trace_rays(o, d, tmax=np.full(len(o), 100.0, np.float32), query=rn.QUERY_ANY)
+    assert (occ[1:] == 0).all()
+    # the finite ray of the batch is answered as usual
+    ref_prim, _, _, _ = sc.trace_rays(o[:1], d[:1])
+    assert prim[0] == ref_prim[0]
+
+
 @pytest.mark.parametrize("builder", ["ploc", "lbvh"])
 def test_closest_and_any_vs_brute(rn, oracle, builder):
     v = soup(3000, seed=5)
