@@ -202,11 +202,11 @@ RN_DEV int packet_unoccluded(const Bvh8Node *__restrict__ nodes, const float4 *_
 					smask[s] |= ((slots >> s) & 1u) << i;
 			}
 		}
-		const uint32_t tri_base = w[5];
+		const uint32_t tri_base = w[5], tmask = w[6];
 		while (tri_any && active) {
 			const uint32_t k = __builtin_ctz(tri_any);
 			tri_any &= tri_any - 1u;
-			const uint32_t ti = tri_base + k;
+			const uint32_t ti = leaf_record(tri_base, tmask, k);
 			const float4 a = gload_f4(tris, 3 * ti), b4 = gload_f4(tris, 3 * ti + 1), c4 = gload_f4(tris, 3 * ti + 2);
 			if (__float_as_int(a.w) == ignore)
 				continue;

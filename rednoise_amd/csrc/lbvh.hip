@@ -18,8 +18,8 @@ namespace {
 #ifndef RN_LEAF_MAX
 #define RN_LEAF_MAX 3
 #endif
-constexpr int kLeafMax = RN_LEAF_MAX; // leaf triangles at most (the meta byte holds 1, 3 or 7)
-static_assert(kLeafMax >= 1 && kLeafMax <= 3, "a leaf's unary count byte holds at most 3 triangles");
+constexpr int kLeafMax = RN_LEAF_MAX; // leaf triangles at most (a node holds three tmask bits per slot)
+static_assert(kLeafMax >= 1 && kLeafMax <= 3, "a slot's tmask bits hold at most 3 triangles");
 constexpr int kBlock = 256;
 
 // Total order on floats as unsigned ints (for atomic min/max of bounds).
@@ -862,9 +862,10 @@ __global__ __launch_bounds__(kCollapseBlock) void k_collapse(int k0, int k1, int
 	nd.e[2] = (uint8_t)ez;
 	nd.child_base = child_base;
 	nd.tri_base = tri_base;
+	nd.tmask = 0;
+	nd.pad = 0;
 #pragma unroll
 	for (int s = 0; s < 8; s++) {
-		nd.meta[s] = 0;
 		for (int a = 0; a < 3; a++) {
 			nd.qlo[a][s] = 255;
 			nd.qhi[a][s] = 0;
@@ -900,7 +901,7 @@ __global__ __launch_bounds__(kCollapseBlock) void k_collapse(int k0, int k1, int
 				task[child_base + ic] = q.id;
 				ic++;
 			} else {
-				nd.meta[s] = (uint8_t)((((1u << cnt) - 1u) << 5) | off);
+				nd.tmask |= ((1u << cnt) - 1u) << (3 * s);
 				// the subtree's (<= kLeafMax = 3) sorted positions, left to
 				// right: an internal child of a 3-triangle subtree has two
 				// leaf children
@@ -963,8 +964,8 @@ __global__ void k_empty_root(Bvh8Node *nodes) {
 			nd.qhi[a][s] = 0;
 		}
 	}
-	for (int s = 0; s < 8; s++)
-		nd.meta[s] = 0;
+	nd.tmask = 0;
+	nd.pad = 0;
 	nd.imask = 0;
 	nd.child_base = 0;
 	nd.tri_base = 0;

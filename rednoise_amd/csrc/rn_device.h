@@ -124,11 +124,10 @@ RN_DEV void fill_oct_table(uint8_t *tab, int tid, int nthreads) {
 
 // Intersect the (up to) 8 child boxes of a node. Returns the internal-child hit
 // bits in traversal order (bit b <-> slot b ^ oct, so bit 0 is the slot built
-// for the ray's own octant, i.e. the nearest) and the leaf triangle bits
-// (bit k <-> triangle tri_base + k). A leaf slot's meta byte holds its
-// triangle count as a unary mask (1, 3 or 7) in bits 5-7 and its offset from
-// tri_base in bits 0-4, so its triangle bits are the mask shifted by the
-// offset; internal and empty slots have meta 0.
+// for the ray's own octant, i.e. the nearest) and the leaf triangle bits in
+// the node's tmask layout (bits 3s .. 3s + 2 for slot s; leaf_record maps a
+// bit to its triangle). A hit slot sets all three of its bits, a constant
+// per slot, and the node's tmask keeps those of its triangles.
 RN_DEV void intersect_children(const uint32_t w[20], const TRay &r, float tmax, uint32_t &ihits, uint32_t &thits,
                                const uint8_t *octab = nullptr) {
 	float px = __uint_as_float(w[0]), py = __uint_as_float(w[1]), pz = __uint_as_float(w[2]);
@@ -143,7 +142,7 @@ RN_DEV void intersect_children(const uint32_t w[20], const TRay &r, float tmax, 
 	uint32_t imask = e >> 24;
 	float ax = (px - r.o.x) * r.inv.x, ay = (py - r.o.y) * r.inv.y, az = (pz - r.o.z) * r.inv.z;
 	float bx = sx * r.inv.x, by = sy * r.inv.y, bz = sz * r.inv.z;
-	// words: 4 child_base, 5 tri_base, 6-7 meta, 8-9 qlo.x, 10-11 qlo.y,
+	// words: 4 child_base, 5 tri_base, 6 tmask, 7 pad, 8-9 qlo.x, 10-11 qlo.y,
 	// 12-13 qlo.z, 14-15 qhi.x, 16-17 qhi.y, 18-19 qhi.z
 	uint32_t nx0 = r.inv.x >= 0 ? w[8] : w[14], nx1 = r.inv.x >= 0 ? w[9] : w[15];
 	uint32_t fx0 = r.inv.x >= 0 ? w[14] : w[8], fx1 = r.inv.x >= 0 ? w[15] : w[9];
@@ -164,7 +163,6 @@ RN_DEV void intersect_children(const uint32_t w[20], const TRay &r, float tmax, 
 #pragma unroll
 		for (int k = 0; k < 2; k++) {
 			const int s = 2 * h + k;
-			const int sh = (s & 3) * 8;
 			float tnx = fmaf((float)qnx[k], bx, ax);
 			float tny = fmaf((float)qny[k], by, ay);
 			float tnz = fmaf((float)qnz[k], bz, az);
@@ -174,14 +172,16 @@ RN_DEV void intersect_children(const uint32_t w[20], const TRay &r, float tmax, 
 			float tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, 0.0f));
 			float tf = fminf(fminf(tfx, tfy), fminf(tfz, tfar_lim));
 			bool hit = tn <= tf * 1.0000005f;
-			const uint32_t mw = s < 4 ? w[6] : w[7];
-			const uint32_t unary = (mw >> (sh + 5)) & 7u;
-			const uint32_t bits = (unary << ((mw >> sh) & 31u)) | (1u << (24 + s));
-			acc |= hit ? bits : 0u;
+			acc |= hit ? (7u << (3 * s)) | (1u << (24 + s)) : 0u;
 		}
 	}
 	ihits = octab ? (uint32_t)octab[(r.oct << 8) | ((acc >> 24) & imask)] : permute_oct((acc >> 24) & imask, r.oct);
-	thits = acc & 0xFFFFFFu;
+	thits = acc & w[6]; // tmask has no bits above 23
+}
+
+// The leaf-ordered triangle record of bit k of a node's tmask layout.
+RN_DEV uint32_t leaf_record(uint32_t tri_base, uint32_t tmask, uint32_t k) {
+	return tri_base + __builtin_popcount(tmask & ((1u << k) - 1u));
 }
 
 // -------------------------------------------------------- triangle tests --
@@ -384,8 +384,8 @@ struct Trav {
 	TRay r;
 	WRay wr;
 	uint64_t grp;
-	uint32_t tri_base, thits;
-	uint32_t tri_base2, thits2; // a second pending leaf group (kTriSlots 2)
+	uint32_t tri_base, thits, tmask;
+	uint32_t tri_base2, thits2, tmask2; // a second pending leaf group (kTriSlots 2)
 	int sp;
 	Hit h;
 	uint64_t spill[SPILL > 0 ? SPILL : 1];
@@ -411,8 +411,10 @@ RN_DEV void trav_init(Trav<SPILL> &s, float3 o, float3 d, float tmax) {
 	s.grp = finite ? mk_group(0u, 1u, 1u << (0u ^ s.r.oct)) : 0ull;
 	s.tri_base = 0;
 	s.thits = 0;
+	s.tmask = 0;
 	s.tri_base2 = 0;
 	s.thits2 = 0;
+	s.tmask2 = 0;
 }
 
 // Triangles a lane tests per step (0 = all of the node's hit leaves at
@@ -466,9 +468,11 @@ RN_DEV bool trav_step(Trav<SPILL> &s, const Bvh8Node *__restrict__ nodes, const 
 		if (kTriSlots == 1 || s.thits == 0) {
 			s.thits = th;
 			s.tri_base = w[5];
+			s.tmask = w[6];
 		} else {
 			s.thits2 = th;
 			s.tri_base2 = w[5];
+			s.tmask2 = w[6];
 		}
 	}
 	if (s.thits) {
@@ -477,7 +481,7 @@ RN_DEV bool trav_step(Trav<SPILL> &s, const Bvh8Node *__restrict__ nodes, const 
 		while (m && (kTriBudget == 0 || budget-- > 0)) {
 			uint32_t k = __builtin_ctz(m);
 			m &= m - 1u;
-			uint32_t ti = s.tri_base + k;
+			uint32_t ti = leaf_record(s.tri_base, s.tmask, k);
 			// The id test does not branch before the triangle test: a branch
 			// there lets the compiler sink the second and third loads behind
 			// the first, which costs a second memory round trip per test.
@@ -514,6 +518,7 @@ RN_DEV bool trav_step(Trav<SPILL> &s, const Bvh8Node *__restrict__ nodes, const 
 		if (kTriSlots == 2 && m == 0) {
 			s.thits = s.thits2;
 			s.tri_base = s.tri_base2;
+			s.tmask = s.tmask2;
 			s.thits2 = 0;
 		}
 	}

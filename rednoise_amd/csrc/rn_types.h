@@ -11,9 +11,12 @@
 // Compressed Wide BVHs" (HPG 2017).
 //   imask bit s   : slot s is an internal child; internal children are stored
 //                   contiguously from child_base in slot order.
-//   meta[s]       : leaf slot: unary triangle count (1, 3 or 7 for one, two
-//                   or three triangles) << 5 | offset from tri_base (0..21);
-//                   internal or empty slot: 0.
+//   tmask         : leaf triangles, three bits per slot: bit 3s + j is set
+//                   when slot s is a leaf of more than j triangles (none for
+//                   an internal or empty slot). Leaves follow each other in
+//                   slot order from tri_base, so the triangle of bit k is
+//                   record tri_base + (number of tmask bits below bit k).
+//   pad           : zero.
 //   empty slot    : qlo = 255, qhi = 0 on every axis (never hit).
 // Slot s is assigned to the child lying furthest along the diagonal whose
 // component k is +1 when bit k of s is set and -1 otherwise. A ray with sign
@@ -25,7 +28,8 @@ struct alignas(16) Bvh8Node {
 	uint8_t imask;
 	uint32_t child_base;
 	uint32_t tri_base;
-	uint8_t meta[8];
+	uint32_t tmask;
+	uint32_t pad;
 	uint8_t qlo[3][8];
 	uint8_t qhi[3][8];
 };

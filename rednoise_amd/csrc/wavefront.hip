@@ -403,6 +403,15 @@ constexpr unsigned kChunksPerWave = RN_CHUNKS_PER_WAVE, kChunkMin = RN_CHUNK_MIN
 constexpr int kRefillMin = RN_REFILL_MIN;
 static_assert(kRefillMin >= 1 && kRefillMin <= 64, "refill threshold is a lane count");
 
+// What the single-scene trace instance keeps of its scene (unused by the
+// batch instance, whose lanes read their own scene's at each refill).
+struct TraceScene {
+	const Bvh8Node *nodes;
+	const float4 *tris;
+	float *out;
+	float4 eye;
+};
+
 // Persistent trace over a queue. QUERY 0: closest hit of path-queue entries
 // [0, cur_count + gen_count), written to q.hit; dead entries are skipped.
 // QUERY 1: any hit of shadow-queue entries [0, sh_count); unoccluded
@@ -413,7 +422,7 @@ static_assert(kRefillMin >= 1 && kRefillMin <= 64, "refill threshold is a lane c
 // all 64 lanes reach each __ballot/__shfl together. The loop ends once the
 // chunk counter has passed the queue length and every lane is idle.
 template <int QUERY, int LDS, int SPILL, int BS, bool ONE>
-RN_DEV void trace_queue(const WQ &q, int cur, uint64_t *stk, const uint8_t *octab) {
+RN_DEV void trace_queue(const WQ &q, int cur, uint64_t *stk, const uint8_t *octab, const TraceScene &ts) {
 	WCtl *c = q.ctl;
 	const unsigned n = QUERY == 0 ? c->cur_count + c->gen_count : c->sh_pending;
 	unsigned *ctrs = QUERY == 0 ? c->trace_ctr : c->shadow_ctr;
@@ -441,16 +450,13 @@ RN_DEV void trace_queue(const WQ &q, int cur, uint64_t *stk, const uint8_t *octa
 	float3 fin_c = f3(0.f, 0.f, 0.f);
 	// ONE: every queued ray belongs to scene c->single, whose BVH, triangles
 	// and frame buffer are held in scalar registers
-	const int one = ONE ? c->single : -1;
-	const Bvh8Node *nodes = ONE ? q.scenes[one].sc.nodes : nullptr;
-	const float4 *tris = ONE ? q.scenes[one].sc.tris : nullptr;
-	float *const out_one = ONE ? q.scenes[one].out : nullptr;
+	const Bvh8Node *nodes = ts.nodes;
+	const float4 *tris = ts.tris;
+	float *const out_one = ts.out;
 	// compact camera paths (entries from cur_count on) start at the eye
 	const bool compact = compact_camera_paths(q);
 	const unsigned ncur = c->cur_count;
-	const float4 eye =
-	    ONE ? make_float4(q.scenes[one].cam.eye[0], q.scenes[one].cam.eye[1], q.scenes[one].cam.eye[2], 0.f)
-	        : make_float4(0.f, 0.f, 0.f, 0.f);
+	const float4 eye = ts.eye;
 	Trav<SPILL> t;
 	unsigned nrays = 0; // this lane's rays (at most the queue length, below 2^31)
 	while (true) {
@@ -588,8 +594,18 @@ __global__ __launch_bounds__(BS) RN_TRACE_ATTR void k_trace(WQ q, int cur) {
 		fill_oct_table(octab, threadIdx.x, BS);
 		__syncthreads();
 	}
-	trace_queue<0, LDS, SPILL, BS, ONE>(q, cur, stk, RN_OCT_TABLE ? octab : nullptr);
-	trace_queue<1, LDS, SPILL, BS, ONE>(q, 0, stk, RN_OCT_TABLE ? octab : nullptr);
+	// the single scene's pointers and camera, read once before any store (so
+	// into scalar registers) for both queries
+	TraceScene ts{nullptr, nullptr, nullptr, make_float4(0.f, 0.f, 0.f, 0.f)};
+	if (ONE) {
+		const cWScene &w = ((const cWScene *)q.scenes)[q.ctl->single];
+		ts.nodes = w.sc.nodes;
+		ts.tris = w.sc.tris;
+		ts.out = (float *)w.out;
+		ts.eye = make_float4(w.cam.eye[0], w.cam.eye[1], w.cam.eye[2], 0.f);
+	}
+	trace_queue<0, LDS, SPILL, BS, ONE>(q, cur, stk, RN_OCT_TABLE ? octab : nullptr, ts);
+	trace_queue<1, LDS, SPILL, BS, ONE>(q, 0, stk, RN_OCT_TABLE ? octab : nullptr, ts);
 }
 
 // Shade every queued path at its closest hit: emission (camera rays only),

@@ -65,7 +65,16 @@ def test_lbvh_stages_match_oracle(rn, oracle, n, dup):
 
 
 NODE = np.dtype([("p", "<f4", 3), ("e", "u1", 3), ("imask", "u1"), ("child_base", "<u4"), ("tri_base", "<u4"),
-                 ("meta", "u1", 8), ("qlo", "u1", (3, 8)), ("qhi", "u1", (3, 8))])
+                 ("tmask", "<u4"), ("pad", "<u4"), ("qlo", "u1", (3, 8)), ("qhi", "u1", (3, 8))])
+
+
+def leaf_of(nd, s):
+    """(first leaf-ordered record, triangle count) of slot s: three tmask
+    bits per slot, leaves consecutive in slot order from tri_base."""
+    t = int(nd["tmask"])
+    bits = (t >> (3 * s)) & 7
+    assert bits in (0, 1, 3, 7), "a slot's triangle bits are a unary count"
+    return int(nd["tri_base"]) + bin(t & ((1 << (3 * s)) - 1)).count("1"), bin(bits).count("1")
 
 
 def check_bvh(verts, nodes_raw, prims, width):
@@ -87,8 +96,9 @@ def check_bvh(verts, nodes_raw, prims, width):
         ic = 0
         for s in range(8):
             internal = (nd["imask"] >> s) & 1
-            meta = int(nd["meta"][s])
-            if not internal and meta == 0:
+            first, cnt = leaf_of(nd, s)
+            assert not (internal and cnt), "an internal slot has no triangles"
+            if not internal and cnt == 0:
                 continue
             nch += 1
             blo = nd["p"].astype(np.float64) + nd["qlo"][:, s] * scale
@@ -99,10 +109,7 @@ def check_bvh(verts, nodes_raw, prims, width):
                 clo, chi, cd = visit(c, depth + 1)
                 depth_c = cd
             else:
-                unary, off = meta >> 5, meta & 31
-                assert unary in (1, 3, 7), "leaf count is a unary mask"
-                cnt = bin(unary).count("1")
-                idx = prims[int(nd["tri_base"]) + off: int(nd["tri_base"]) + off + cnt]
+                idx = prims[first: first + cnt]
                 clo = vmin[idx].min(0)
                 chi = vmax[idx].max(0)
             assert (blo <= clo).all() and (bhi >= chi).all(), "quantised child box must contain its child"
@@ -126,16 +133,14 @@ def canonical_tree(nodes_raw, prims):
         ic = 0
         for s in range(8):
             internal = (nd["imask"] >> s) & 1
-            meta = int(nd["meta"][s])
-            if not internal and meta == 0:
+            b, cnt = leaf_of(nd, s)
+            if not internal and cnt == 0:
                 continue
             box = (tuple(nd["qlo"][:, s].tolist()), tuple(nd["qhi"][:, s].tolist()))
             if internal:
                 out.append((s, box, canon(int(nd["child_base"]) + ic)))
                 ic += 1
             else:
-                off, cnt = meta & 31, bin(meta >> 5).count("1")
-                b = int(nd["tri_base"]) + off
                 out.append((s, box, tuple(prims[b:b + cnt].tolist())))
         return tuple(out)
 
@@ -171,7 +176,7 @@ def test_flat_child_on_grid_line(rn):
     assert nd["imask"] == 0
     sy = np.float32(2.0) ** (int(nd["e"][1]) - 127)
     assert sy == np.float32(1 / 16)
-    flat = [s for s in range(8) if nd["meta"][s] and prims[int(nd["tri_base"]) + (nd["meta"][s] & 31)] == 0]
+    flat = [s for s in range(8) if leaf_of(nd, s)[1] and prims[leaf_of(nd, s)[0]] == 0]
     assert len(flat) == 1
     s = flat[0]
     assert nd["qlo"][1, s] == nd["qhi"][1, s]
