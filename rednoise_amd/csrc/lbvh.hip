@@ -456,136 +456,254 @@ __global__ __launch_bounds__(kBlock) void k_ploc_nn(const uint32_t *pm, const fl
 	nn[i] = bj;
 }
 
-// Flags of the clusters [0, m); the zeros written over [m, m_ub) keep the
-// scans over the host's bound m_ub exact.
-__global__ void k_ploc_flags(const uint32_t *pm, int m_ub, const int *nn, uint32_t *mflag, uint32_t *kflag) {
-	int i = blockIdx.x * blockDim.x + threadIdx.x;
-	const int m = (int)*pm;
-	if (i >= m_ub)
-		return;
-	if (i >= m || m <= 1) {
-		mflag[i] = 0u;
-		kflag[i] = 0u;
-		return;
-	}
-	int j = nn[i];
-	bool mutual = j >= 0 && nn[j] == i;
-	mflag[i] = (mutual && i < j) ? 1u : 0u;
-	kflag[i] = (mutual && i > j) ? 0u : 1u;
+// A cluster's flags: mf when it merges with its nearest neighbour and has
+// the smaller index of the pair (it then owns the new node), kf when it
+// keeps a place in the next round (every cluster except the larger index
+// of a merging pair).
+__device__ __forceinline__ void ploc_flags(const int *nn, int i, uint32_t &mf, uint32_t &kf) {
+	const int j = nn[i];
+	const bool mutual = j >= 0 && nn[j] == i;
+	mf = (mutual && i < j) ? 1u : 0u;
+	kf = (mutual && i > j) ? 0u : 1u;
 }
 
 // Round state on the device, so that rounds run without the host:
-// tot[0] merges and tot[1] clusters kept by the round, tot[2] set when a
-// round with m > 1 merged nothing, tot[3] m, tot[4] id_top.
-constexpr int kPlocMerges = 0, kPlocKept = 1, kPlocErr = 2, kPlocM = 3, kPlocTop = 4;
+// tot[0] merges and tot[1] clusters kept by the last round, tot[2] set when a
+// round with m > 1 merged nothing, tot[3] m, tot[4] id_top (the next
+// internal id, counting down), tot[5] and tot[6] the m and id_top the
+// current round's merge works from.
+constexpr int kPlocMerges = 0, kPlocKept = 1, kPlocErr = 2, kPlocM = 3, kPlocTop = 4, kPlocM0 = 5, kPlocTop0 = 6;
+constexpr int kPlocStateWords = 7;
 
-__global__ void k_ploc_totals(const uint32_t *mflag, const uint32_t *moff, const uint32_t *kflag,
-                              const uint32_t *koff, uint32_t *tot) {
-	const int m = (int)tot[kPlocM];
-	tot[kPlocMerges] = m > 1 ? moff[m - 1] + mflag[m - 1] : 0u;
-	tot[kPlocKept] = m > 1 ? koff[m - 1] + kflag[m - 1] : (uint32_t)m;
-}
+// A round after the neighbour search is three kernels: the merge and keep
+// counts of each tile of kPlocTile clusters, one work-group's scan of the
+// tile counts (which also advances the round state), and the merge, which
+// recomputes its tile's flags and their offsets. Each thread of the tile
+// kernels takes kPlocTile / kBlock consecutive clusters.
+constexpr int kPlocTile = 4 * kBlock;
+constexpr int kPlocPer = kPlocTile / kBlock;
+constexpr int kPlocScanBlock = 1024;
 
-__global__ void k_ploc_advance(uint32_t *tot) {
-	const int m = (int)tot[kPlocM];
-	if (m <= 1)
-		return;
-	if (tot[kPlocMerges] == 0u || tot[kPlocKept] != (uint32_t)m - tot[kPlocMerges])
-		tot[kPlocErr] = 1u;
-	tot[kPlocTop] = (uint32_t)((int)tot[kPlocTop] - (int)tot[kPlocMerges]);
-	tot[kPlocM] = tot[kPlocKept];
-}
-
-__global__ void k_ploc_merge(const uint32_t *tot, const int *nn, const uint32_t *mflag, const uint32_t *moff,
-                             const uint32_t *kflag, const uint32_t *koff, const int *cref, const float4 *cbox,
-                             const uint32_t *ccnt, int *cref2, float4 *cbox2, uint32_t *ccnt2, int *left, int *right,
-                             uint32_t *count, float4 *box) {
-	int i = blockIdx.x * blockDim.x + threadIdx.x;
-	const int m = (int)tot[kPlocM], id_top = (int)tot[kPlocTop];
-	if (m <= 1 || i >= m || !kflag[i])
-		return;
-	const uint32_t pos = koff[i];
-	float4 lo = cbox[2 * (size_t)i], hi = cbox[2 * (size_t)i + 1];
-	if (mflag[i]) {
-		const int j = nn[i];
-		const int id = id_top - (int)moff[i];
-		float4 blo = cbox[2 * (size_t)j], bhi = cbox[2 * (size_t)j + 1];
-		lo = make_float4(fminf(lo.x, blo.x), fminf(lo.y, blo.y), fminf(lo.z, blo.z), 0.0f);
-		hi = make_float4(fmaxf(hi.x, bhi.x), fmaxf(hi.y, bhi.y), fmaxf(hi.z, bhi.z), 0.0f);
-		left[id] = cref[i];
-		right[id] = cref[j];
-		const uint32_t c = ccnt[i] + ccnt[j];
-		count[id] = c;
-		box[2 * (size_t)id] = lo;
-		box[2 * (size_t)id + 1] = hi;
-		cref2[pos] = id;
-		ccnt2[pos] = c;
-	} else {
-		cref2[pos] = cref[i];
-		ccnt2[pos] = ccnt[i];
+// Exclusive scan of a pair of counts over the block (blockDim.x threads, a
+// multiple of 64); *total gets the block's sums. One call per kernel.
+template <int BS>
+__device__ __forceinline__ uint2 block_scan2(uint2 v, uint2 *total) {
+	__shared__ uint2 wsum[BS / 64];
+	const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+	uint2 x = v;
+#pragma unroll
+	for (int off = 1; off < 64; off <<= 1) {
+		const uint32_t a = __shfl_up(x.x, off), b = __shfl_up(x.y, off);
+		if (lane >= off) {
+			x.x += a;
+			x.y += b;
+		}
 	}
-	cbox2[2 * (size_t)pos] = lo;
-	cbox2[2 * (size_t)pos + 1] = hi;
+	if (lane == 63)
+		wsum[w] = x;
+	__syncthreads();
+	uint2 pre = make_uint2(0u, 0u), tot = make_uint2(0u, 0u);
+	for (int k = 0; k < BS / 64; k++) {
+		const uint2 s = wsum[k];
+		if (k < w) {
+			pre.x += s.x;
+			pre.y += s.y;
+		}
+		tot.x += s.x;
+		tot.y += s.y;
+	}
+	*total = tot;
+	return make_uint2(pre.x + x.x - v.x, pre.y + x.y - v.y);
+}
+
+// Per tile: (merges, kept) of its clusters below m.
+__global__ __launch_bounds__(kBlock) void k_ploc_count(const uint32_t *tot, const int *nn, uint2 *tsum) {
+	const int m = (int)tot[kPlocM];
+	const int t0 = blockIdx.x * kPlocTile;
+	if (m <= 1 || t0 >= m)
+		return;
+	uint32_t sm = 0, sk = 0;
+#pragma unroll
+	for (int r = 0; r < kPlocPer; r++) {
+		const int i = t0 + r * kBlock + threadIdx.x;
+		if (i < m) {
+			uint32_t mf, kf;
+			ploc_flags(nn, i, mf, kf);
+			sm += mf;
+			sk += kf;
+		}
+	}
+	uint2 total;
+	block_scan2<kBlock>(make_uint2(sm, sk), &total);
+	if (threadIdx.x == 0)
+		tsum[blockIdx.x] = total;
+}
+
+// Exclusive offsets of the tiles' counts, then the round's totals: the
+// current (m, id_top) are kept for the merge and the next round's are set.
+__global__ __launch_bounds__(kPlocScanBlock) void k_ploc_scan(uint32_t *tot, const uint2 *tsum, uint2 *toff) {
+	const int m = (int)tot[kPlocM], id_top = (int)tot[kPlocTop];
+	if (m <= 1) {
+		if (threadIdx.x == 0) {
+			tot[kPlocM0] = (uint32_t)m;
+			tot[kPlocTop0] = (uint32_t)id_top;
+		}
+		return;
+	}
+	const int nt = (m + kPlocTile - 1) / kPlocTile;
+	const int per = (nt + kPlocScanBlock - 1) / kPlocScanBlock;
+	const int b0 = min(nt, (int)threadIdx.x * per), b1 = min(nt, b0 + per);
+	uint2 s = make_uint2(0u, 0u);
+	for (int t = b0; t < b1; t++) {
+		s.x += tsum[t].x;
+		s.y += tsum[t].y;
+	}
+	uint2 total;
+	uint2 o = block_scan2<kPlocScanBlock>(s, &total);
+	for (int t = b0; t < b1; t++) {
+		toff[t] = o;
+		o.x += tsum[t].x;
+		o.y += tsum[t].y;
+	}
+	if (threadIdx.x == 0) {
+		tot[kPlocM0] = (uint32_t)m;
+		tot[kPlocTop0] = (uint32_t)id_top;
+		tot[kPlocMerges] = total.x;
+		tot[kPlocKept] = total.y;
+		if (total.x == 0u || total.y != (uint32_t)m - total.x)
+			tot[kPlocErr] = 1u;
+		tot[kPlocTop] = (uint32_t)(id_top - (int)total.x);
+		tot[kPlocM] = total.y;
+	}
+}
+
+// The merge: a kept cluster moves to its keep offset; a merging pair's
+// owner writes internal node id_top - (its merge offset), with the pair's
+// union box, and the new cluster refers to it.
+__global__ __launch_bounds__(kBlock) void k_ploc_merge(const uint32_t *tot, const int *nn, const uint2 *toff,
+                                                       const int *cref, const float4 *cbox, const uint32_t *ccnt,
+                                                       int *cref2, float4 *cbox2, uint32_t *ccnt2, int *left,
+                                                       int *right, uint32_t *count, float4 *box) {
+	const int m = (int)tot[kPlocM0], id_top = (int)tot[kPlocTop0];
+	const int t0 = blockIdx.x * kPlocTile;
+	if (m <= 1 || t0 >= m)
+		return;
+	const int i0 = t0 + kPlocPer * threadIdx.x;
+	uint32_t fm = 0, fk = 0;
+#pragma unroll
+	for (int r = 0; r < kPlocPer; r++) {
+		if (i0 + r < m) {
+			uint32_t mf, kf;
+			ploc_flags(nn, i0 + r, mf, kf);
+			fm |= mf << r;
+			fk |= kf << r;
+		}
+	}
+	uint2 total;
+	const uint2 ex = block_scan2<kBlock>(make_uint2((uint32_t)__builtin_popcount(fm), (uint32_t)__builtin_popcount(fk)),
+	                                     &total);
+	uint32_t mo = toff[blockIdx.x].x + ex.x, ko = toff[blockIdx.x].y + ex.y;
+#pragma unroll
+	for (int r = 0; r < kPlocPer; r++) {
+		const int i = i0 + r;
+		if ((fk >> r) & 1u) {
+			const uint32_t pos = ko;
+			float4 lo = cbox[2 * (size_t)i], hi = cbox[2 * (size_t)i + 1];
+			if ((fm >> r) & 1u) {
+				const int j = nn[i];
+				const int id = id_top - (int)mo;
+				const float4 blo = cbox[2 * (size_t)j], bhi = cbox[2 * (size_t)j + 1];
+				lo = make_float4(fminf(lo.x, blo.x), fminf(lo.y, blo.y), fminf(lo.z, blo.z), 0.0f);
+				hi = make_float4(fmaxf(hi.x, bhi.x), fmaxf(hi.y, bhi.y), fmaxf(hi.z, bhi.z), 0.0f);
+				left[id] = cref[i];
+				right[id] = cref[j];
+				const uint32_t c = ccnt[i] + ccnt[j];
+				count[id] = c;
+				box[2 * (size_t)id] = lo;
+				box[2 * (size_t)id + 1] = hi;
+				cref2[pos] = id;
+				ccnt2[pos] = c;
+			} else {
+				cref2[pos] = cref[i];
+				ccnt2[pos] = ccnt[i];
+			}
+			cbox2[2 * (size_t)pos] = lo;
+			cbox2[2 * (size_t)pos + 1] = hi;
+		}
+		mo += (fm >> r) & 1u;
+		ko += (fk >> r) & 1u;
+	}
 }
 
 // The last rounds of PLOC, from kPlocTail clusters down to the root, in one
 // work-group: the same rounds as the kernels above (same windows, pairs,
 // ids and order), with block barriers in place of kernel boundaries, so the
 // tree is the same and the ~10 launches and one host round trip of each
-// small round are saved. On entry: m clusters in buffer `cur`; st[0] = m,
-// st[1] = id_top (the next internal id, counting down). On exit st[0] = 1 and
-// st[1] = -1 when every cluster has merged.
+// small round are saved. The clusters' boxes and nearest neighbours stay in
+// LDS for all of these rounds (the neighbour search reads 2 * kPlocR boxes
+// per cluster); their references and triangle counts stay in the two global
+// buffers. On entry: m clusters in buffer `cur`; st[0] = m, st[1] = id_top
+// (the next internal id, counting down). On exit st[0] = 1 and st[1] = -1
+// when every cluster has merged. The cluster buffers hold no boxes of these
+// rounds afterwards (nothing reads them: the tree is left, right, count and
+// box).
 constexpr int kPlocTailBlock = 1024;
 #ifndef RN_PLOC_TAIL
-#define RN_PLOC_TAIL 8192
+#define RN_PLOC_TAIL 4096
 #endif
-constexpr int kPlocTail = RN_PLOC_TAIL; // at most 8 clusters per thread
+constexpr int kPlocTail = RN_PLOC_TAIL;
+constexpr int kPlocTailPer = kPlocTail / kPlocTailBlock; // clusters per thread at most
+static_assert(kPlocTail % kPlocTailBlock == 0 && kPlocTailPer <= 8, "a thread merges at most 8 clusters");
+static_assert(kPlocTail * 36 + kPlocTailBlock * 8 <= 160 * 1024, "the tail's LDS fits a work-group");
 
-__global__ __launch_bounds__(kPlocTailBlock) void k_ploc_tail(int n, int cur, int *cref, float4 *cbox,
-                                                                uint32_t *ccnt, int *nn, uint32_t *mflag,
-                                                                uint32_t *kflag, int *left, int *right,
+__global__ __launch_bounds__(kPlocTailBlock) void k_ploc_tail(int n, int cur, int *cref, const float4 *cbox,
+                                                                uint32_t *ccnt, int *left, int *right,
                                                                 uint32_t *count, float4 *box, int *st) {
+	__shared__ float4 sbox[2 * kPlocTail]; // lo, hi of cluster i at 2i, 2i + 1
+	__shared__ int snn[kPlocTail];
 	__shared__ uint32_t sm[kPlocTailBlock], sk[kPlocTailBlock];
 	const int tid = threadIdx.x;
 	int m = st[0], id_top = st[1];
+	if (m > kPlocTail)
+		return; // the host calls the tail at m <= kPlocTail; st[0] > 1 reports it
+	for (int i = tid; i < 2 * m; i += kPlocTailBlock)
+		sbox[i] = cbox[(size_t)cur * 2 * n + i];
+	__syncthreads();
 	while (m > 1) {
 		const int *cr = cref + (size_t)cur * n;
-		const float4 *cb = cbox + (size_t)cur * 2 * n;
 		const uint32_t *cc = ccnt + (size_t)cur * n;
 		int *cr2 = cref + (size_t)(cur ^ 1) * n;
-		float4 *cb2 = cbox + (size_t)(cur ^ 1) * 2 * n;
 		uint32_t *cc2 = ccnt + (size_t)(cur ^ 1) * n;
 		// nearest neighbours (k_ploc_nn)
 		for (int i = tid; i < m; i += kPlocTailBlock) {
-			const float4 alo = cb[2 * (size_t)i], ahi = cb[2 * (size_t)i + 1];
+			const float4 alo = sbox[2 * i], ahi = sbox[2 * i + 1];
 			float best = INFINITY;
 			int bj = -1;
 			const int j0 = max(0, i - kPlocR), j1 = min(m - 1, i + kPlocR);
 			for (int j = j0; j <= j1; j++) {
 				if (j == i)
 					continue;
-				float a = merged_area(alo, ahi, cb[2 * (size_t)j], cb[2 * (size_t)j + 1]);
+				float a = merged_area(alo, ahi, sbox[2 * j], sbox[2 * j + 1]);
 				if (a < best) {
 					best = a;
 					bj = j;
 				}
 			}
-			nn[i] = bj;
+			snn[i] = bj;
 		}
 		__syncthreads();
-		// flags (k_ploc_flags) over this thread's contiguous chunk, and its sums
+		// flags (ploc_flags) over this thread's contiguous chunk (bit r for
+		// cluster b0 + r), and its sums
 		const int per = (m + kPlocTailBlock - 1) / kPlocTailBlock;
 		const int b0 = min(m, tid * per), b1 = min(m, b0 + per);
-		uint32_t tm = 0, tk = 0;
+		uint32_t fm = 0, fk = 0;
 		for (int i = b0; i < b1; i++) {
-			const int j = nn[i];
-			const bool mutual = j >= 0 && nn[j] == i;
-			const uint32_t mf = (mutual && i < j) ? 1u : 0u, kf = (mutual && i > j) ? 0u : 1u;
-			mflag[i] = mf;
-			kflag[i] = kf;
-			tm += mf;
-			tk += kf;
+			const int j = snn[i];
+			const bool mutual = j >= 0 && snn[j] == i;
+			fm |= (mutual && i < j) ? 1u << (i - b0) : 0u;
+			fk |= (mutual && i > j) ? 0u : 1u << (i - b0);
 		}
+		const uint32_t tm = (uint32_t)__builtin_popcount(fm), tk = (uint32_t)__builtin_popcount(fk);
 		sm[tid] = tm;
 		sk[tid] = tk;
 		__syncthreads();
@@ -597,17 +715,22 @@ __global__ __launch_bounds__(kPlocTailBlock) void k_ploc_tail(int n, int cur, in
 			__syncthreads();
 		}
 		const uint32_t merges = sm[kPlocTailBlock - 1], kept = sk[kPlocTailBlock - 1];
-		// merge (k_ploc_merge) with this thread's exclusive offsets
+		// merge (k_ploc_merge) with this thread's exclusive offsets; the new
+		// boxes wait in registers until every thread has read the old ones
 		uint32_t mo = sm[tid] - tm, ko = sk[tid] - tk;
-		for (int i = b0; i < b1; i++) {
-			const uint32_t mf = mflag[i], kf = kflag[i];
-			if (kf) {
+		float4 nlo[kPlocTailPer], nhi[kPlocTailPer];
+#pragma unroll
+		for (int r = 0; r < kPlocTailPer; r++) {
+			const int i = b0 + r;
+			const bool kf = (fk >> r) & 1u, mf = (fm >> r) & 1u;
+			nlo[r] = nhi[r] = make_float4(0.f, 0.f, 0.f, 0.f);
+			if (i < b1 && kf) {
 				const uint32_t pos = ko;
-				float4 lo = cb[2 * (size_t)i], hi = cb[2 * (size_t)i + 1];
+				float4 lo = sbox[2 * i], hi = sbox[2 * i + 1];
 				if (mf) {
-					const int j = nn[i];
+					const int j = snn[i];
 					const int id = id_top - (int)mo;
-					const float4 blo = cb[2 * (size_t)j], bhi = cb[2 * (size_t)j + 1];
+					const float4 blo = sbox[2 * j], bhi = sbox[2 * j + 1];
 					lo = make_float4(fminf(lo.x, blo.x), fminf(lo.y, blo.y), fminf(lo.z, blo.z), 0.0f);
 					hi = make_float4(fmaxf(hi.x, bhi.x), fmaxf(hi.y, bhi.y), fmaxf(hi.z, bhi.z), 0.0f);
 					left[id] = cr[i];
@@ -622,11 +745,23 @@ __global__ __launch_bounds__(kPlocTailBlock) void k_ploc_tail(int n, int cur, in
 					cr2[pos] = cr[i];
 					cc2[pos] = cc[i];
 				}
-				cb2[2 * (size_t)pos] = lo;
-				cb2[2 * (size_t)pos + 1] = hi;
+				nlo[r] = lo;
+				nhi[r] = hi;
 			}
-			mo += mf;
-			ko += kf;
+			if (i < b1) {
+				mo += mf;
+				ko += kf;
+			}
+		}
+		__syncthreads();
+		ko = sk[tid] - tk;
+#pragma unroll
+		for (int r = 0; r < kPlocTailPer; r++) {
+			if (b0 + r < b1 && ((fk >> r) & 1u)) {
+				sbox[2 * ko] = nlo[r];
+				sbox[2 * ko + 1] = nhi[r];
+				ko++;
+			}
 		}
 		__syncthreads();
 		if (merges == 0u || (int)kept != m - (int)merges)
@@ -996,10 +1131,10 @@ void ploc_tree(Bvh &b, const float *d_verts, int n, hipStream_t s) {
 	float4 *cbox = static_cast<float4 *>(b.ploc_box.reserve((size_t)n * 2 * 32));
 	uint32_t *ccnt = static_cast<uint32_t *>(b.ploc_cnt.reserve((size_t)n * 2 * 4));
 	int *nn = static_cast<int *>(b.ploc_nn.reserve((size_t)n * 4));
-	uint32_t *fl = static_cast<uint32_t *>(b.ploc_flags.reserve(((size_t)n * 4 + 8) * 4));
-	uint32_t *mflag = fl, *kflag = fl + n, *moff = fl + 2 * (size_t)n, *koff = fl + 3 * (size_t)n,
-	         *tot = fl + 4 * (size_t)n;
-	uint32_t *tmp = static_cast<uint32_t *>(b.scan_tmp.reserve(scan_tmp_words((size_t)n) * 4 + 16));
+	const size_t nt_max = ((size_t)n + kPlocTile - 1) / kPlocTile;
+	uint32_t *fl = static_cast<uint32_t *>(b.ploc_flags.reserve((4 * nt_max + 8) * 4));
+	uint2 *tsum = reinterpret_cast<uint2 *>(fl), *toff = reinterpret_cast<uint2 *>(fl + 2 * nt_max);
+	uint32_t *tot = fl + 4 * nt_max;
 	k_leaf_boxes<<<grid_for(n), kBlock, 0, s>>>(d_verts, order, n, box, cref, cbox, ccnt);
 	RN_HIP(hipGetLastError());
 	// Rounds run in batches of kPlocBatch with m and id_top on the device
@@ -1010,12 +1145,11 @@ void ploc_tree(Bvh &b, const float *d_verts, int n, hipStream_t s) {
 	constexpr int kPlocBatch = 4;
 	int m_ub = n, cur = 0;
 	uint32_t hs[8] = {0u, 0u, 0u, (uint32_t)n, (uint32_t)(n - 2), 0u, 0u, 0u};
-	RN_HIP(hipMemcpyAsync(tot, hs, 5 * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+	RN_HIP(hipMemcpyAsync(tot, hs, kPlocStateWords * sizeof(uint32_t), hipMemcpyHostToDevice, s));
 	while (m_ub > 1) {
 		if (m_ub <= kPlocTail) { // the rest in one work-group
 			int *st = reinterpret_cast<int *>(tot + kPlocM);
-			k_ploc_tail<<<1, kPlocTailBlock, 0, s>>>(n, cur, cref, cbox, ccnt, nn, mflag, kflag, left, right, count,
-			                                         box, st);
+			k_ploc_tail<<<1, kPlocTailBlock, 0, s>>>(n, cur, cref, cbox, ccnt, left, right, count, box, st);
 			RN_HIP(hipGetLastError());
 			break;
 		}
@@ -1023,14 +1157,11 @@ void ploc_tree(Bvh &b, const float *d_verts, int n, hipStream_t s) {
 			int *cr = cref + (size_t)cur * n, *cr2 = cref + (size_t)(cur ^ 1) * n;
 			float4 *cb = cbox + (size_t)cur * 2 * n, *cb2 = cbox + (size_t)(cur ^ 1) * 2 * n;
 			uint32_t *cc = ccnt + (size_t)cur * n, *cc2 = ccnt + (size_t)(cur ^ 1) * n;
+			const unsigned nt = grid_for(m_ub, kPlocTile);
 			k_ploc_nn<<<grid_for(m_ub), kBlock, 0, s>>>(tot + kPlocM, cb, nn);
-			k_ploc_flags<<<grid_for(m_ub), kBlock, 0, s>>>(tot + kPlocM, m_ub, nn, mflag, kflag);
-			exclusive_scan(mflag, moff, (size_t)m_ub, tmp, s);
-			exclusive_scan(kflag, koff, (size_t)m_ub, tmp, s);
-			k_ploc_totals<<<1, 1, 0, s>>>(mflag, moff, kflag, koff, tot);
-			k_ploc_merge<<<grid_for(m_ub), kBlock, 0, s>>>(tot, nn, mflag, moff, kflag, koff, cr, cb, cc, cr2, cb2,
-			                                            cc2, left, right, count, box);
-			k_ploc_advance<<<1, 1, 0, s>>>(tot);
+			k_ploc_count<<<nt, kBlock, 0, s>>>(tot, nn, tsum);
+			k_ploc_scan<<<1, kPlocScanBlock, 0, s>>>(tot, tsum, toff);
+			k_ploc_merge<<<nt, kBlock, 0, s>>>(tot, nn, toff, cr, cb, cc, cr2, cb2, cc2, left, right, count, box);
 			cur ^= 1;
 		}
 		RN_HIP(hipMemcpyAsync(hs, tot, 5 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
