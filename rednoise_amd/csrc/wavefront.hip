@@ -500,13 +500,16 @@ RN_DEV void trace_queue(const WQ &q, int cur, uint64_t *stk, const uint8_t *octa
 					const bool cam = QUERY == 0 && ONE && compact && my >= ncur;
 					const float4 d4 = QUERY == 0 ? get4(D + my) : D[my];
 					const float4 o4 = cam ? eye : (QUERY == 0 ? get4(O + my) : O[my]);
-					// closest hit: the path's meta word; any hit: the
+					// closest hit: the path's meta word (a one-scene render's
+					// paths below cur_count, pushed by the shade stage, are
+					// live: their meta is not read); any hit: the
 					// contribution and scene index
 					const float4 m4 = QUERY == 1 ? M[my]
 					                 : make_float4(0.f, 0.f, 0.f,
 					                               cam ? __uint_as_float(__float_as_uint(d4.w) == kDeadSample
 					                                                         ? kDead << 16 : 0u)
-					                                   : M[my].w);
+					                               : (ONE && compact) ? 0.f
+					                                                  : M[my].w);
 					const uint32_t meta = __float_as_uint(m4.w);
 					if (QUERY == 1 || (meta >> 16) != kDead) {
 						if (!ONE) {
