@@ -484,11 +484,11 @@ constexpr int kPlocTile = 4 * kBlock;
 constexpr int kPlocPer = kPlocTile / kBlock;
 constexpr int kPlocScanBlock = 1024;
 
-// Exclusive scan of a pair of counts over the block (blockDim.x threads, a
-// multiple of 64); *total gets the block's sums. One call per kernel.
+// Exclusive scan of a pair of counts over a block of BS threads (a multiple
+// of 64); *total gets the block's sums. `wsum` holds BS / 64 entries; a
+// second call on the same array needs a barrier between the two.
 template <int BS>
-__device__ __forceinline__ uint2 block_scan2(uint2 v, uint2 *total) {
-	__shared__ uint2 wsum[BS / 64];
+__device__ __forceinline__ uint2 block_scan2(uint2 v, uint2 *total, uint2 *wsum) {
 	const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 	uint2 x = v;
 #pragma unroll
@@ -533,8 +533,9 @@ __global__ __launch_bounds__(kBlock) void k_ploc_count(const uint32_t *tot, cons
 			sk += kf;
 		}
 	}
+	__shared__ uint2 wsum[kBlock / 64];
 	uint2 total;
-	block_scan2<kBlock>(make_uint2(sm, sk), &total);
+	block_scan2<kBlock>(make_uint2(sm, sk), &total, wsum);
 	if (threadIdx.x == 0)
 		tsum[blockIdx.x] = total;
 }
@@ -558,8 +559,9 @@ __global__ __launch_bounds__(kPlocScanBlock) void k_ploc_scan(uint32_t *tot, con
 		s.x += tsum[t].x;
 		s.y += tsum[t].y;
 	}
+	__shared__ uint2 wsum[kPlocScanBlock / 64];
 	uint2 total;
-	uint2 o = block_scan2<kPlocScanBlock>(s, &total);
+	uint2 o = block_scan2<kPlocScanBlock>(s, &total, wsum);
 	for (int t = b0; t < b1; t++) {
 		toff[t] = o;
 		o.x += tsum[t].x;
@@ -579,39 +581,35 @@ __global__ __launch_bounds__(kPlocScanBlock) void k_ploc_scan(uint32_t *tot, con
 
 // The merge: a kept cluster moves to its keep offset; a merging pair's
 // owner writes internal node id_top - (its merge offset), with the pair's
-// union box, and the new cluster refers to it.
+// union box, and the new cluster refers to it. Pass r of a tile takes its
+// clusters t0 + r * kBlock + threadIdx.x (coalesced), so each pass scans
+// its flags and adds the counts of the passes before it.
 __global__ __launch_bounds__(kBlock) void k_ploc_merge(const uint32_t *tot, const int *nn, const uint2 *toff,
                                                        const int *cref, const float4 *cbox, const uint32_t *ccnt,
                                                        int *cref2, float4 *cbox2, uint32_t *ccnt2, int *left,
                                                        int *right, uint32_t *count, float4 *box) {
+	__shared__ uint2 wsum[2][kBlock / 64];
 	const int m = (int)tot[kPlocM0], id_top = (int)tot[kPlocTop0];
 	const int t0 = blockIdx.x * kPlocTile;
 	if (m <= 1 || t0 >= m)
 		return;
-	const int i0 = t0 + kPlocPer * threadIdx.x;
-	uint32_t fm = 0, fk = 0;
+	uint2 base = toff[blockIdx.x];
 #pragma unroll
 	for (int r = 0; r < kPlocPer; r++) {
-		if (i0 + r < m) {
-			uint32_t mf, kf;
-			ploc_flags(nn, i0 + r, mf, kf);
-			fm |= mf << r;
-			fk |= kf << r;
-		}
-	}
-	uint2 total;
-	const uint2 ex = block_scan2<kBlock>(make_uint2((uint32_t)__builtin_popcount(fm), (uint32_t)__builtin_popcount(fk)),
-	                                     &total);
-	uint32_t mo = toff[blockIdx.x].x + ex.x, ko = toff[blockIdx.x].y + ex.y;
-#pragma unroll
-	for (int r = 0; r < kPlocPer; r++) {
-		const int i = i0 + r;
-		if ((fk >> r) & 1u) {
-			const uint32_t pos = ko;
+		const int i = t0 + r * kBlock + threadIdx.x;
+		uint32_t mf = 0, kf = 0;
+		if (i < m)
+			ploc_flags(nn, i, mf, kf);
+		uint2 total;
+		// alternate the scan's LDS: a pass's writes must not meet the
+		// previous pass's reads
+		const uint2 ex = block_scan2<kBlock>(make_uint2(mf, kf), &total, wsum[r & 1]);
+		if (kf) {
+			const uint32_t pos = base.y + ex.y;
 			float4 lo = cbox[2 * (size_t)i], hi = cbox[2 * (size_t)i + 1];
-			if ((fm >> r) & 1u) {
+			if (mf) {
 				const int j = nn[i];
-				const int id = id_top - (int)mo;
+				const int id = id_top - (int)(base.x + ex.x);
 				const float4 blo = cbox[2 * (size_t)j], bhi = cbox[2 * (size_t)j + 1];
 				lo = make_float4(fminf(lo.x, blo.x), fminf(lo.y, blo.y), fminf(lo.z, blo.z), 0.0f);
 				hi = make_float4(fmaxf(hi.x, bhi.x), fmaxf(hi.y, bhi.y), fmaxf(hi.z, bhi.z), 0.0f);
@@ -630,8 +628,8 @@ __global__ __launch_bounds__(kBlock) void k_ploc_merge(const uint32_t *tot, cons
 			cbox2[2 * (size_t)pos] = lo;
 			cbox2[2 * (size_t)pos + 1] = hi;
 		}
-		mo += (fm >> r) & 1u;
-		ko += (fk >> r) & 1u;
+		base.x += total.x;
+		base.y += total.y;
 	}
 }
 
@@ -654,14 +652,14 @@ constexpr int kPlocTailBlock = 1024;
 constexpr int kPlocTail = RN_PLOC_TAIL;
 constexpr int kPlocTailPer = kPlocTail / kPlocTailBlock; // clusters per thread at most
 static_assert(kPlocTail % kPlocTailBlock == 0 && kPlocTailPer <= 8, "a thread merges at most 8 clusters");
-static_assert(kPlocTail * 36 + kPlocTailBlock * 8 <= 160 * 1024, "the tail's LDS fits a work-group");
+static_assert(kPlocTail * 36 + kPlocTailBlock / 8 <= 160 * 1024, "the tail's LDS fits a work-group");
 
 __global__ __launch_bounds__(kPlocTailBlock) void k_ploc_tail(int n, int cur, int *cref, const float4 *cbox,
                                                                 uint32_t *ccnt, int *left, int *right,
                                                                 uint32_t *count, float4 *box, int *st) {
 	__shared__ float4 sbox[2 * kPlocTail]; // lo, hi of cluster i at 2i, 2i + 1
 	__shared__ int snn[kPlocTail];
-	__shared__ uint32_t sm[kPlocTailBlock], sk[kPlocTailBlock];
+	__shared__ uint2 wsum[kPlocTailBlock / 64];
 	const int tid = threadIdx.x;
 	int m = st[0], id_top = st[1];
 	if (m > kPlocTail)
@@ -703,21 +701,13 @@ __global__ __launch_bounds__(kPlocTailBlock) void k_ploc_tail(int n, int cur, in
 			fm |= (mutual && i < j) ? 1u << (i - b0) : 0u;
 			fk |= (mutual && i > j) ? 0u : 1u << (i - b0);
 		}
-		const uint32_t tm = (uint32_t)__builtin_popcount(fm), tk = (uint32_t)__builtin_popcount(fk);
-		sm[tid] = tm;
-		sk[tid] = tk;
-		__syncthreads();
-		for (int off = 1; off < kPlocTailBlock; off <<= 1) {
-			const uint32_t xm = tid >= off ? sm[tid - off] : 0u, xk = tid >= off ? sk[tid - off] : 0u;
-			__syncthreads();
-			sm[tid] += xm;
-			sk[tid] += xk;
-			__syncthreads();
-		}
-		const uint32_t merges = sm[kPlocTailBlock - 1], kept = sk[kPlocTailBlock - 1];
+		uint2 total;
+		const uint2 ex = block_scan2<kPlocTailBlock>(
+		    make_uint2((uint32_t)__builtin_popcount(fm), (uint32_t)__builtin_popcount(fk)), &total, wsum);
+		const uint32_t merges = total.x, kept = total.y;
 		// merge (k_ploc_merge) with this thread's exclusive offsets; the new
 		// boxes wait in registers until every thread has read the old ones
-		uint32_t mo = sm[tid] - tm, ko = sk[tid] - tk;
+		uint32_t mo = ex.x, ko = ex.y;
 		float4 nlo[kPlocTailPer], nhi[kPlocTailPer];
 #pragma unroll
 		for (int r = 0; r < kPlocTailPer; r++) {
@@ -754,7 +744,7 @@ __global__ __launch_bounds__(kPlocTailBlock) void k_ploc_tail(int n, int cur, in
 			}
 		}
 		__syncthreads();
-		ko = sk[tid] - tk;
+		ko = ex.y;
 #pragma unroll
 		for (int r = 0; r < kPlocTailPer; r++) {
 			if (b0 + r < b1 && ((fk >> r) & 1u)) {
