@@ -562,14 +562,15 @@ RN_DEV void trace_queue(const WQ &q, int cur, uint64_t *stk, const uint8_t *octa
 }
 
 // Occupancy target of the trace kernels without a spill stack, in waves per
-// SIMD: 6 fits the single-scene instances in 80 VGPRs with no scratch (2%
-// faster at 1M triangles than the compiler's 87 VGPRs and 5 waves); 0 = the
-// compiler's choice.
+// SIMD: 6 fits both instances in 80 VGPRs with no scratch (the single-scene
+// one 2% faster at 1M triangles than at the compiler's 87 VGPRs and 5 waves,
+// the batch one 1.4% on 64 scenes against its 95 VGPRs); 0 = the compiler's
+// choice.
 #ifndef RN_TRACE_WPE
 #define RN_TRACE_WPE 6
 #endif
-#define RN_TRACE_ATTR __attribute__((amdgpu_waves_per_eu((SPILL == 0 && ONE) ? RN_TRACE_WPE : 0, \
-                                                         (SPILL == 0 && ONE) ? RN_TRACE_WPE : 0)))
+#define RN_TRACE_ATTR \
+	__attribute__((amdgpu_waves_per_eu(SPILL == 0 ? RN_TRACE_WPE : 0, SPILL == 0 ? RN_TRACE_WPE : 0)))
 
 // The octant permutation of a node's hit mask as a table in LDS (one byte
 // load per node visit instead of three conditional bit swaps).
