@@ -29,7 +29,9 @@ RN_DEV float3 xyz(float4 a) { return f3(a.x, a.y, a.z); }
 
 // ---------------------------------------------------------------- Philox --
 // Philox4x32-10 (Salmon et al., SC'11): ten rounds of two 32x32->64-bit
-// multiplies; mul_hi/mul_lo map to v_mul_hi_u32 / v_mul_lo_u32.
+// multiplies. Each product is one v_mad_u64_u32 (high and low halves
+// together; v_mul_hi_u32 + v_mul_lo_u32 would be two quarter-rate
+// instructions) and each three-way xor one v_bitop3_b32 (truth table 0x96).
 
 struct u32x4 {
 	uint32_t x, y, z, w;
@@ -42,13 +44,13 @@ RN_DEV u32x4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, u
 			k0 += 0x9E3779B9u;
 			k1 += 0xBB67AE85u;
 		}
-		uint32_t hi0 = __umulhi(0xD2511F53u, c0), lo0 = 0xD2511F53u * c0;
-		uint32_t hi1 = __umulhi(0xCD9E8D57u, c2), lo1 = 0xCD9E8D57u * c2;
-		uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+		const uint64_t p0 = (uint64_t)0xD2511F53u * c0, p1 = (uint64_t)0xCD9E8D57u * c2;
+		const uint32_t n0 = __builtin_amdgcn_bitop3_b32((uint32_t)(p1 >> 32), c1, k0, 0x96);
+		const uint32_t n2 = __builtin_amdgcn_bitop3_b32((uint32_t)(p0 >> 32), c3, k1, 0x96);
 		c0 = n0;
-		c1 = lo1;
+		c1 = (uint32_t)p1;
 		c2 = n2;
-		c3 = lo0;
+		c3 = (uint32_t)p0;
 	}
 	return u32x4{c0, c1, c2, c3};
 }
