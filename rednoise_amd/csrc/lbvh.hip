@@ -479,7 +479,7 @@ constexpr int kPlocStateWords = 7;
 // counts of each tile of kPlocTile clusters, one work-group's scan of the
 // tile counts (which also advances the round state), and the merge, which
 // recomputes its tile's flags and their offsets. Each thread of the tile
-// kernels takes kPlocTile / kBlock consecutive clusters.
+// kernels takes kPlocTile / kBlock clusters of its tile, kBlock apart.
 constexpr int kPlocTile = 4 * kBlock;
 constexpr int kPlocPer = kPlocTile / kBlock;
 constexpr int kPlocScanBlock = 1024;
@@ -636,8 +636,8 @@ __global__ __launch_bounds__(kBlock) void k_ploc_merge(const uint32_t *tot, cons
 // The last rounds of PLOC, from kPlocTail clusters down to the root, in one
 // work-group: the same rounds as the kernels above (same windows, pairs,
 // ids and order), with block barriers in place of kernel boundaries, so the
-// tree is the same and the ~10 launches and one host round trip of each
-// small round are saved. The clusters' boxes and nearest neighbours stay in
+// tree is the same and the four launches (and a share of a host round
+// trip) of each small round are saved. The clusters' boxes and nearest neighbours stay in
 // LDS for all of these rounds (the neighbour search reads 2 * kPlocR boxes
 // per cluster); their references and triangle counts stay in the two global
 // buffers. On entry: m clusters in buffer `cur`; st[0] = m, st[1] = id_top
