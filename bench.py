@@ -216,10 +216,14 @@ def batch_bench(rn, torch):
     rn.render_batch(items, opts)
     dt_first = time.perf_counter() - t0
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    imgs = rn.render_batch(items, opts)
-    dt = time.perf_counter() - t0
-    st = rn.last_stats()
+    # medians of 3 calls (a call's host side, 64 image downloads, varies)
+    runs = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        imgs = rn.render_batch(items, opts)
+        runs.append((time.perf_counter() - t0, rn.last_stats()))
+    runs.sort(key=lambda r: r[0])
+    dt, st = runs[1]
     rays = st["rays_closest"] + st["rays_shadow"]
     # the same scenes, one wavefront call each (same options)
     for sc, cam, p in items:
@@ -227,14 +231,18 @@ def batch_bench(rn, torch):
         p.queue_paths, p.block_size, p.bvh_width = opts.queue_paths, opts.block_size, opts.bvh_width
     for sc, cam, p in items[:2]:
         sc.render_hdr(cam, p)
-    t0 = time.perf_counter()
-    each_rays, each_gpu = 0, 0.0
-    for sc, cam, p in items:
-        sc.render_hdr(cam, p)
-        s2 = rn.last_stats()
-        each_rays += s2["rays_closest"] + s2["rays_shadow"]
-        each_gpu += s2["render_ms"] * 1e-3
-    dt_each = time.perf_counter() - t0
+    passes = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        each_rays, each_gpu = 0, 0.0
+        for sc, cam, p in items:
+            sc.render_hdr(cam, p)
+            s2 = rn.last_stats()
+            each_rays += s2["rays_closest"] + s2["rays_shadow"]
+            each_gpu += s2["render_ms"] * 1e-3
+        passes.append((time.perf_counter() - t0, each_rays, each_gpu))
+    passes.sort(key=lambda r: r[0])
+    dt_each, each_rays, each_gpu = passes[1]
     for sc, cam, p in items:  # the per-scene megakernel path runs on width-8 BVHs, built here
         p.integrator, p.queue_paths, p.block_size, p.bvh_width = rn.INTEGRATOR_MEGAKERNEL, 0, 0, 8
         sc.build(8)
@@ -270,7 +278,8 @@ def batch_bench(rn, torch):
     assert each_rays == rays == plain_rays, (rays, each_rays, plain_rays)
     return {"workload": f"config2: 64 scenes, {sum(sc.triangle_count for sc, _, _ in items):,} triangles, "
                         f"{sum(p.width * p.height * p.spp for _, _, p in items) / 1e6:.1f}M paths, depth 8, "
-                        "one rn_render_batch call: persistent wavefront launches replayed from a cached hipGraph",
+                        "one rn_render_batch call: persistent wavefront launches replayed from a cached hipGraph "
+                        "(times: medians of 3 calls, and of 3 passes of one call per scene)",
             "batched_mrays_s": round(rays / dt / 1e6, 2), "batched_s": round(dt, 3),
             "batched_gpu_s": round(st["render_ms"] * 1e-3, 3), "batched_first_call_s": round(dt_first, 3),
             "iterations": st["iterations"],
