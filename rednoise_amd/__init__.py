@@ -371,6 +371,16 @@ def _f32(a, n=None) -> np.ndarray:
     return a
 
 
+def _out_f32(a, n: int) -> np.ndarray:
+    """A caller's output array (C-contiguous float32 of n values, viewed flat),
+    or a new one."""
+    if a is None:
+        return np.empty(n, np.float32)
+    if not isinstance(a, np.ndarray) or a.dtype != np.float32 or not a.flags.c_contiguous or a.size != n:
+        raise ValueError(f"out must be a C-contiguous float32 array of {n} values")
+    return a.reshape(-1)
+
+
 # ------------------------------------------------------------------ image --
 
 class Image:
@@ -547,10 +557,12 @@ class Scene:
             raise _err("rn_render_bidirectional")
         return Image(buf.tobytes(), width, height)
 
-    def render_hdr(self, camera: Camera, params: RenderParams) -> np.ndarray:
-        """``rn_render_ex``: (H, W, 3) float32 mean radiance."""
+    def render_hdr(self, camera: Camera, params: RenderParams, out: Optional[np.ndarray] = None) -> np.ndarray:
+        """``rn_render_ex``: (H, W, 3) float32 mean radiance, into `out` when
+        given (a C-contiguous float32 array of width*height*3 values, reused
+        across calls without new host pages)."""
         self._check_open()
-        out = np.empty(params.width * params.height * 3, np.float32)
+        out = _out_f32(out, params.width * params.height * 3)
         if _load_library().rn_render_ex(self._handle, C.byref(camera), C.byref(params), _ptr(out)) != 1:
             raise _err("rn_render_ex")
         return out.reshape(params.height, params.width, 3)
@@ -687,13 +699,17 @@ class Scene:
         return f"<rednoise.Scene {self._n} triangles>" if self._handle else "<rednoise.Scene (closed)>"
 
 
-def render_batch(items: Sequence[Tuple[Scene, Camera, RenderParams]], options: Optional[BatchOptions] = None):
-    """``rn_render_batch``: list of (H, W, 3) images, one per item."""
+def render_batch(items: Sequence[Tuple[Scene, Camera, RenderParams]], options: Optional[BatchOptions] = None,
+                 out: Optional[Sequence[np.ndarray]] = None):
+    """``rn_render_batch``: list of (H, W, 3) images, one per item, into the
+    arrays of `out` when given (as for ``Scene.render_hdr``)."""
     lib = _load_library()
+    if out is not None and len(out) != len(items):
+        raise ValueError("out must hold one array per item")
     outs = []
     arr = (BatchItem * len(items))()
     for k, (sc, cam, p) in enumerate(items):
-        o = np.empty(p.width * p.height * 3, np.float32)
+        o = _out_f32(None if out is None else out[k], p.width * p.height * 3)
         outs.append(o)
         arr[k].scene = sc.handle
         arr[k].camera = cam

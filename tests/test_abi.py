@@ -207,3 +207,22 @@ def test_reference_import_names():
             img.save("x.jpg")
         with pytest.raises(ImportError):
             img.to_pillow()
+
+
+def test_output_arrays_are_checked_before_the_call(rn):
+    """render_hdr / render_batch write into caller arrays only when those are
+    C-contiguous float32 of width*height*3 values; a mismatch raises before
+    the library is called."""
+    import numpy as np
+    v = np.array([[[-1, -1, 3], [1, -1, 3], [0, 1, 3]]], np.float32)
+    sc = rn.Scene.from_arrays(v, np.full((1, 3), 0.5, np.float32))
+    cam = rn.Camera.make((0, 0, 0), (0, 0, 1))
+    p = rn.RenderParams.make(8, 4, 1, 2)
+    for bad in (np.empty(8 * 4 * 3 - 1, np.float32), np.empty(8 * 4 * 3, np.float64),
+                np.empty((8 * 4 * 3, 2), np.float32)[:, 0]):
+        with pytest.raises(ValueError):
+            sc.render_hdr(cam, p, out=bad)
+        with pytest.raises(ValueError):
+            rn.render_batch([(sc, cam, p)], out=[bad])
+    with pytest.raises(ValueError):
+        rn.render_batch([(sc, cam, p)], out=[])

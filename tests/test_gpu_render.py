@@ -307,3 +307,20 @@ def test_rn_render_pathtraced_past_the_seed_overflow(rn):
     past = ys * W + xs > 219_553
     print("identical before / past the overflow:", (d[~past] == 0).mean(), (d[past] == 0).mean(), "max", d.max())
     assert (d[past] == 0).mean() > 0.999 and (d[~past] == 0).mean() > 0.999 and (d <= 2).all()
+
+
+def test_render_into_caller_arrays(rn):
+    """render_hdr and render_batch fill caller arrays (reused across calls)
+    with the images they return otherwise."""
+    items = _items(rn)
+    opts = rn.BatchOptions(0, 0, 8, 1, 8)
+    ref = rn.render_batch(items, opts)
+    outs = [np.full(p.width * p.height * 3, -1.0, np.float32) for _, _, p in items]
+    for _ in range(2):
+        got = rn.render_batch(items, opts, out=outs)
+        for g, r, o in zip(got, ref, outs):
+            assert np.array_equal(g, r) and np.shares_memory(g, o)
+    sc, cam, p = items[0]
+    a = sc.render_hdr(cam, p)
+    b = sc.render_hdr(cam, p, out=outs[0])
+    assert np.array_equal(a, b) and np.shares_memory(b, outs[0])
