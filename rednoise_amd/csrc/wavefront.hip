@@ -35,6 +35,19 @@ namespace {
 
 constexpr int kPixTile = 8; // 8x8 pixel tiles in generation order
 
+// Unsigned division by a divisor fixed for a render, as a multiply-high,
+// two shifts and an add: Granlund and Montgomery, "Division by invariant
+// integers using multiplication" (PLDI 1994), figure 4.1, exact for every
+// 32-bit dividend. m = floor(2^32 (2^l - d) / d) + 1 with l = ceil(log2 d),
+// s1 = min(l, 1), s2 = max(l - 1, 0); set on the host (make_fastdiv).
+struct FastDiv {
+	uint32_t m, s1, s2;
+};
+__device__ __forceinline__ uint32_t fdiv(uint32_t n, const FastDiv &f) {
+	const uint32_t t = __umulhi(n, f.m);
+	return (t + ((n - t) >> f.s1)) >> f.s2;
+}
+
 struct WScene {
 	DevScene sc;
 	DevCamera cam;
@@ -42,6 +55,7 @@ struct WScene {
 	float *out;
 	unsigned long long first, count; // global work range
 	int tiles_x, npix_pad;
+	FastDiv fd_width, fd_tiles, fd_npix; // division by p.width, tiles_x, npix_pad
 };
 
 // Chunk counters of the trace kernels are striped over kStripes addresses,
@@ -283,19 +297,11 @@ RN_DEV bool compact_camera_paths(const WQ &q) { return q.nscenes == 1; }
 // they carry. The host keeps the work range below 2^32 in slot mode.
 struct SlotMap {
 	uint32_t first, width, tiles_x, npix_pad, sbase;
+	FastDiv fd_width, fd_tiles;
 };
 RN_DEV SlotMap slot_map(const WScene &ws) {
 	return SlotMap{(uint32_t)ws.first, (uint32_t)ws.p.width, (uint32_t)ws.tiles_x, (uint32_t)ws.npix_pad,
-	               ws.p.sample_base};
-}
-// The map of a scene the whole launch works on, read once into scalar
-// registers (a load inside the shade loop would be a vector load: the
-// loop's stores could alias the scene table as far as the compiler knows).
-RN_DEV SlotMap slot_map_uniform(const WScene &ws) {
-	const SlotMap m = slot_map(ws);
-	return SlotMap{(uint32_t)__builtin_amdgcn_readfirstlane(m.first), (uint32_t)__builtin_amdgcn_readfirstlane(m.width),
-	               (uint32_t)__builtin_amdgcn_readfirstlane(m.tiles_x),
-	               (uint32_t)__builtin_amdgcn_readfirstlane(m.npix_pad), (uint32_t)__builtin_amdgcn_readfirstlane(m.sbase)};
+	               ws.p.sample_base, ws.fd_width, ws.fd_tiles};
 }
 // The pixel of the camera path of work index g and sample smp (the inverse
 // of k_generate's tiling: smp is the sample plane, the remainder the tiled
@@ -303,11 +309,12 @@ RN_DEV SlotMap slot_map_uniform(const WScene &ws) {
 RN_DEV uint32_t camera_pixel(const SlotMap &m, unsigned long long g, uint32_t smp) {
 	const uint32_t pp = (uint32_t)(g - m.first - (unsigned long long)(smp - m.sbase) * m.npix_pad);
 	const uint32_t tile = pp >> 6, t = pp & 63u;
-	const uint32_t x = (tile % m.tiles_x) * kPixTile + (t & 7u), y = (tile / m.tiles_x) * kPixTile + (t >> 3);
+	const uint32_t ty = fdiv(tile, m.fd_tiles), tx = tile - ty * m.tiles_x;
+	const uint32_t x = tx * kPixTile + (t & 7u), y = ty * kPixTile + (t >> 3);
 	return y * m.width + x;
 }
 RN_DEV uint32_t path_slot(const SlotMap &m, uint32_t pix, uint32_t smp) {
-	const uint32_t y = pix / m.width, x = pix - y * m.width;
+	const uint32_t y = fdiv(pix, m.fd_width), x = pix - y * m.width;
 	const uint32_t pp = ((((y >> 3) * m.tiles_x) + (x >> 3)) << 6) | ((y & 7u) << 3) | (x & 7u);
 	return m.first + (smp - m.sbase) * m.npix_pad + pp;
 }
@@ -350,15 +357,17 @@ __global__ __launch_bounds__(256) void k_generate(WQ q, int cur, const WScene *_
 		unsigned long long local = g - ws.first;
 		uint32_t smp, pp;
 		if ((local >> 32) == 0) { // 32-bit division for work ranges below 2^32
-			smp = ws.p.sample_base + (uint32_t)local / (uint32_t)ws.npix_pad;
-			pp = (uint32_t)local % (uint32_t)ws.npix_pad;
+			const uint32_t s = fdiv((uint32_t)local, ws.fd_npix);
+			smp = ws.p.sample_base + s;
+			pp = (uint32_t)local - s * (uint32_t)ws.npix_pad;
 		} else {
 			smp = ws.p.sample_base + (uint32_t)(local / (unsigned long long)ws.npix_pad);
 			pp = (uint32_t)(local % (unsigned long long)ws.npix_pad);
 		}
 		uint32_t tile = pp >> 6, t = pp & 63u;
-		int x = (int)(tile % (uint32_t)ws.tiles_x) * kPixTile + (int)(t & 7u);
-		int y = (int)(tile / (uint32_t)ws.tiles_x) * kPixTile + (int)(t >> 3);
+		const uint32_t ty = fdiv(tile, ws.fd_tiles);
+		int x = (int)(tile - ty * (uint32_t)ws.tiles_x) * kPixTile + (int)(t & 7u);
+		int y = (int)ty * kPixTile + (int)(t >> 3);
 		float4 o4 = make_float4(0.f, 0.f, 0.f, 0.f), d4 = o4;
 		uint32_t meta = (uint32_t)si | (kDead << 16);
 		if (x < ws.p.width && y < ws.p.height) {
@@ -633,13 +642,13 @@ __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(ONE
 	if (instance_skips<ONE>(q))
 		return;
 	const unsigned one = ONE ? (unsigned)q.ctl->single : 0u;
-	const SlotMap m1 = ONE && q.acc ? slot_map_uniform(scenes[one]) : SlotMap{};
 	const bool compact = compact_camera_paths(q);
-	// the camera and the map of a compact camera path's work index, kept in
-	// LDS (scalar registers are at their limit for two blocks per CU)
+	// the camera and the scene's slot map (a compact camera path's pixel, a
+	// path's radiance slot), kept in LDS: scalar registers are at their limit
+	// for two blocks per CU
 	__shared__ float cam_eye[3];
 	__shared__ SlotMap cam_map;
-	if (ONE && compact) {
+	if (ONE) {
 		if (threadIdx.x < 3)
 			cam_eye[threadIdx.x] = scenes[one].cam.eye[threadIdx.x];
 		if (threadIdx.x == 0)
@@ -715,9 +724,11 @@ __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(ONE
 							atomic_add_global(fb + 2, beta.z * le.z);
 						}
 					} else if (q.acc) {
-						aux = path_slot(ONE ? m1
+						aux = path_slot(ONE ? cam_map
 						                        : SlotMap{(uint32_t)ws.first, (uint32_t)ws.p.width, (uint32_t)ws.tiles_x,
-						                                  (uint32_t)ws.npix_pad, ws.p.sample_base},
+						                                  (uint32_t)ws.npix_pad, ws.p.sample_base,
+						                                  FastDiv{ws.fd_width.m, ws.fd_width.s1, ws.fd_width.s2},
+						                                  FastDiv{ws.fd_tiles.m, ws.fd_tiles.s1, ws.fd_tiles.s2}},
 						                pix, smp);
 					}
 					float3 a = load3(sc.albedo, prim);
@@ -874,6 +885,17 @@ WaveWork &work() {
 
 } // namespace
 
+// The constants of FastDiv for a divisor d >= 1.
+FastDiv make_fastdiv(uint32_t d) {
+	if (d == 0)
+		throw Error("wavefront: division by zero in the work-index map");
+	uint32_t l = 0;
+	while (l < 32 && (1ull << l) < d)
+		l++;
+	const uint64_t m = (((1ull << l) - d) << 32) / d + 1; // (2^l - d) < 2^32: no overflow
+	return FastDiv{(uint32_t)m, l < 1 ? l : 1u, l > 1 ? l - 1 : 0u};
+}
+
 void wavefront_release() {
 	WaveWork &ww = work();
 	// the graphs refer to the queues freed below
@@ -973,6 +995,9 @@ void wavefront_render_on(const WaveItem *items, int count, const WaveConfig &cfg
 		w.tiles_x = (it.params.width + kPixTile - 1) / kPixTile;
 		int tiles_y = (it.params.height + kPixTile - 1) / kPixTile;
 		w.npix_pad = w.tiles_x * tiles_y * kPixTile * kPixTile;
+		w.fd_width = make_fastdiv((uint32_t)it.params.width);
+		w.fd_tiles = make_fastdiv((uint32_t)w.tiles_x);
+		w.fd_npix = make_fastdiv((uint32_t)w.npix_pad);
 		w.first = total;
 		w.count = (unsigned long long)w.npix_pad * (unsigned long long)it.params.spp;
 		total += w.count;
