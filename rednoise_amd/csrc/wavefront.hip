@@ -1115,10 +1115,13 @@ void wavefront_render_on(const WaveItem *items, int count, const WaveConfig &cfg
 	const unsigned sgrid = std::min<unsigned>((unsigned)std::max(1, n_cu) * 8u, (cap + 255) / 256);
 	const unsigned shgrid = std::min<unsigned>((unsigned)std::max(1, n_cu) * (2048u / kShadeBlock),
 	                                           (cap + kShadeBlock - 1) / kShadeBlock);
-	auto iteration = [&](int cur) {
+	// gen: the iteration's k_generate (left out when the queue took the whole
+	// work range in the first iteration: later ones would generate nothing)
+	auto iteration = [&](int cur, bool gen = true) {
 		if (timed && nev == 0)
 			RN_HIP(hipEventRecord(pool_event(ww, nev++), s));
-		k_generate<<<sgrid, 256, 0, s>>>(q, cur, q.scenes);
+		if (gen)
+			k_generate<<<sgrid, 256, 0, s>>>(q, cur, q.scenes);
 		mark(1);
 #define RN_LAUNCH1(K, LDSD, SPD, DEEP, ONE, ...)                                                                      \
 	do {                                                                                                               \
@@ -1225,7 +1228,7 @@ void wavefront_render_on(const WaveItem *items, int count, const WaveConfig &cfg
 			RN_HIP(hipGraphLaunch(iters == 0 ? graph_of(ww.first, n) : graph_of(ww.group, n), s));
 		} else {
 			for (uint64_t it = 0; it < n; it++)
-				iteration((int)(it & 1));
+				iteration((int)(it & 1), iters + it == 0 || total > cap);
 		}
 		iters += n;
 		n = (uint64_t)per;
