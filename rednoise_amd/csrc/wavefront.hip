@@ -1216,11 +1216,12 @@ void wavefront_render_on(const WaveItem *items, int count, const WaveConfig &cfg
 	if (fault_injected("wavefront_iterations"))
 		limit = 0; // the check below fires after the first launch
 	// Iterations enqueued before the done flag is first read: as many as the
-	// last render with the same launch arguments needed (rounded up to even),
-	// else one group; after that, one group per read of the flag.
+	// last render with the same launch arguments needed (with a graph rounded
+	// up to even, since a graph starts and ends on the same queue), else one
+	// group; after that, one group per read of the flag.
 	uint64_t n = (uint64_t)per;
 	if (ww.pred_key == key && ww.pred_iters > 0)
-		n = ww.pred_iters + (ww.pred_iters & 1);
+		n = ww.pred_iters + (cfg.use_graph ? (ww.pred_iters & 1) : 0);
 	while (true) {
 		if (iters > limit)
 			throw Error("wavefront: iteration limit exceeded (queue accounting error)");
@@ -1228,7 +1229,7 @@ void wavefront_render_on(const WaveItem *items, int count, const WaveConfig &cfg
 			RN_HIP(hipGraphLaunch(iters == 0 ? graph_of(ww.first, n) : graph_of(ww.group, n), s));
 		} else {
 			for (uint64_t it = 0; it < n; it++)
-				iteration((int)(it & 1), iters + it == 0 || total > cap);
+				iteration((int)((iters + it) & 1), iters + it == 0 || total > cap);
 		}
 		iters += n;
 		n = (uint64_t)per;

@@ -119,8 +119,8 @@ def test_graph_replays_match_eager(rn):
 @pytest.mark.parametrize("graph", [0, 1])
 def test_first_launch_follows_the_last_render(rn, graph):
     """A wavefront render enqueues as many iterations as the last render with
-    the same launch arguments needed (rounded up to even) before it first
-    reads the done flag. Too many: the extra iterations find the queues empty.
+    the same launch arguments needed (with a graph rounded up to even) before
+    it first reads the done flag. Too many: the extra iterations find the queues empty.
     Too few: it goes on in groups. The images equal the megakernel's either
     way. Roulette depth lives in the scene table, not in the launch
     arguments, so it changes the iterations a render needs under one key."""
@@ -145,16 +145,19 @@ def test_first_launch_follows_the_last_render(rn, graph):
         assert st["iterations"] == needed[rr]
         return st["trace_kernel_launches"]
 
+    def enqueued(k):
+        return k + (k & 1) if graph else k
+
     render(1 << 30)
     long_n = needed[1 << 30]
-    assert render(1 << 30) == long_n + (long_n & 1)  # predicted exactly
+    assert render(1 << 30) == enqueued(long_n)  # predicted exactly
     render(1)  # predicted too many
     short_n = needed[1]
     assert short_n < long_n, (short_n, long_n)
-    assert render(1) == short_n + (short_n & 1)
+    assert render(1) == enqueued(short_n)
     launched = render(1 << 30)  # predicted too few: goes on in groups
-    assert launched >= long_n and (launched - (short_n + (short_n & 1))) % 8 == 0
-    assert render(1 << 30) == long_n + (long_n & 1)
+    assert launched >= long_n and (launched - enqueued(short_n)) % 8 == 0
+    assert render(1 << 30) == enqueued(long_n)
     sc.close()
 
 
