@@ -399,12 +399,15 @@ def test_wavefront_matches_megakernel(rn, oracle):
     assert s_wf["rays_shadow"] == s_mk["rays_shadow"]
 
 
-@pytest.mark.parametrize("shape", [(96, 64, 16, 20000), (97, 61, 5, 6000)])
+@pytest.mark.parametrize("shape", [(96, 64, 16, 20000), (97, 61, 5, 6000), (1, 1, 64, 0), (7, 5, 8, 0),
+                                   (65, 3, 4, 64)])
 def test_wavefront_slots_are_the_megakernel_sums(rn, oracle, shape):
     """With radiance slots (the default) the wavefront adds a path's
     contributions in depth order and a pixel's paths in sample order, as the
     megakernel does: the same bits, on every run, for an image whose size is
-    not a multiple of the 8x8 tile and with a queue that refills many times.
+    not a multiple of the 8x8 tile, with a queue that refills many times, and
+    for images one tile (or one pixel) wide, where the work-index map divides
+    by one.
     With RN_FLAG_FB_ATOMICS (float atomics, arbitrary order) the image agrees
     to float rounding."""
     W, H, spp, queue = shape
