@@ -264,6 +264,14 @@ int rn_scene_export_bvh(const rn_scene *scene, void *nodes, int *leaf_prims);
  * node, value < 0: leaf ~value, i.e. sorted position -value-1). */
 int rn_lbvh_stages(const float *verts, int n, uint32_t *morton_sorted, int *order, int *left, int *right);
 
+/* The default builder's stages on the GPU for n triangles: the sorted
+ * primitive order and the PLOC binary hierarchy (parallel locally-ordered
+ * clustering, search radius 8) in the layout of rn_lbvh_stages, internal ids
+ * 0 .. n-2 with the root at 0. Exposes the tree the BVH8 collapse reads; the
+ * reference builds its BVH on the CPU (src/BVH.cpp:18-59) and has no
+ * counterpart of this stage. */
+int rn_ploc_stages(const float *verts, int n, int *order, int *left, int *right);
+
 /* Stable LSD radix sort of (key, value) pairs on the GPU over the low `bits`
  * key bits. */
 int rn_radix_sort_pairs(uint32_t *keys, uint32_t *values, int n, int bits);

@@ -93,3 +93,104 @@ void or_radix_tree(const uint32_t *sorted_codes, int n, int *left, int *right) {
 		return;
 	split_rec(sorted_codes, 0, n - 1, 0, left, right);
 }
+
+/* PLOC (Meister & Bittner 2018, "Parallel Locally-Ordered Clustering for
+ * Bounding Volume Hierarchy Construction") over the sorted order, written as
+ * a plain sequential loop over rounds. The reference has no such builder; the
+ * rules restated here are the GPU builder's own:
+ *  - a cluster's box is the min/max of its triangles' vertices;
+ *  - its nearest neighbour is the cluster within OR_PLOC_R positions whose
+ *    merged box has the least half-area dx*dy + dy*dz + dz*dx (evaluated as
+ *    fmaf(dz, dx, fmaf(dx, dy, dy * dz))), the first in increasing position
+ *    on equal area;
+ *  - a mutual pair merges; its smaller position owns the new node, whose left
+ *    child is the owner's reference, and the new cluster takes the owner's
+ *    place in the next round's order, other clusters keeping theirs;
+ *  - new internal ids count down from n - 2: a round's owners, in increasing
+ *    position, take id_top, id_top - 1, ...
+ * Returns the number of rounds, or -1 when a round merges nothing. */
+#define OR_PLOC_R 8
+
+static float ploc_area(const float *a, const float *b) {
+	float dx = fmaxf(a[3], b[3]) - fminf(a[0], b[0]);
+	float dy = fmaxf(a[4], b[4]) - fminf(a[1], b[1]);
+	float dz = fmaxf(a[5], b[5]) - fminf(a[2], b[2]);
+	return fmaf(dz, dx, fmaf(dx, dy, dy * dz));
+}
+
+int or_ploc_tree(const float *verts, const int *order, int n, int *left, int *right) {
+	if (n < 2)
+		return 0;
+	int *ref = (int *)malloc(sizeof(int) * 2 * (size_t)n);
+	float *box = (float *)malloc(sizeof(float) * 12 * (size_t)n); /* lo xyz, hi xyz */
+	int *nn = (int *)malloc(sizeof(int) * (size_t)n);
+	int *ref2 = ref + n;
+	float *box2 = box + 6 * (size_t)n;
+	for (int p = 0; p < n; p++) {
+		const float *t = verts + 9 * (size_t)order[p];
+		for (int a = 0; a < 3; a++) {
+			box[6 * p + a] = fminf(fminf(t[a], t[3 + a]), t[6 + a]);
+			box[6 * p + 3 + a] = fmaxf(fmaxf(t[a], t[3 + a]), t[6 + a]);
+		}
+		ref[p] = ~p;
+	}
+	int m = n, id_top = n - 2, rounds = 0;
+	while (m > 1) {
+		for (int i = 0; i < m; i++) {
+			float best = INFINITY;
+			int bj = -1;
+			for (int j = i - OR_PLOC_R; j <= i + OR_PLOC_R; j++) {
+				if (j < 0 || j >= m || j == i)
+					continue;
+				float a = ploc_area(box + 6 * i, box + 6 * j);
+				/* (area, parity of the pair's smaller position, smaller
+				 * position, larger position): candidates come in increasing
+				 * j, so the last two keys increase along the loop */
+				int par = (j < i ? j : i) & 1;
+				int bpar = bj < 0 ? 2 : (bj < i ? bj : i) & 1;
+				if (a < best || (a == best && par < bpar)) {
+					best = a;
+					bj = j;
+				}
+			}
+			nn[i] = bj;
+		}
+		int merges = 0, kept = 0;
+		for (int i = 0; i < m; i++) {
+			int j = nn[i];
+			int mutual = j >= 0 && nn[j] == i;
+			if (mutual && j < i)
+				continue; /* merged into its owner j */
+			float *nb = box2 + 6 * kept;
+			if (mutual) {
+				int id = id_top - merges;
+				left[id] = ref[i];
+				right[id] = ref[j];
+				for (int a = 0; a < 3; a++) {
+					nb[a] = fminf(box[6 * i + a], box[6 * j + a]);
+					nb[3 + a] = fmaxf(box[6 * i + 3 + a], box[6 * j + 3 + a]);
+				}
+				ref2[kept] = id;
+				merges++;
+			} else {
+				for (int a = 0; a < 6; a++)
+					nb[a] = box[6 * i + a];
+				ref2[kept] = ref[i];
+			}
+			kept++;
+		}
+		rounds++;
+		if (merges == 0) {
+			rounds = -1;
+			break;
+		}
+		id_top -= merges;
+		m = kept;
+		int *tr = ref; ref = ref2; ref2 = tr;
+		float *tb = box; box = box2; box2 = tb;
+	}
+	free(ref < ref2 ? ref : ref2);
+	free(box < box2 ? box : box2);
+	free(nn);
+	return rounds;
+}

@@ -136,6 +136,15 @@ void or_morton_codes(const float *verts, int n, uint32_t *codes);
  * [pinned: structural invariants + brute-force range checks in tests.] */
 void or_radix_tree(const uint32_t *sorted_codes, int n, int *left, int *right);
 
+/* PLOC binary tree over the sorted order (order[p] = triangle at sorted
+ * position p), search radius 8, same numbering as or_radix_tree's output
+ * arrays (internal ids 0 .. n-2 with the root at 0, leaves ~position).
+ * Returns the number of clustering rounds (-1: a round merged nothing).
+ * [pinned: hand-worked trees (two pairs, a caterpillar and its mirror,
+ *  duplicates); structural invariants; brute-force mutual-pair check of the
+ *  first round.] */
+int or_ploc_tree(const float *verts, const int *order, int n, int *left, int *right);
+
 /* -------------------------------------------------- physical estimator -- */
 
 /* Physical estimator ("Mode P"): Lambertian albedo/pi, NEE to emissive

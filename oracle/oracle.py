@@ -97,6 +97,8 @@ def lib():
         L.or_morton3.restype = C.c_uint32
         L.or_morton_codes.argtypes = [f32p, C.c_int, u32p]
         L.or_radix_tree.argtypes = [u32p, C.c_int, i32p, i32p]
+        L.or_ploc_tree.argtypes = [f32p, i32p, C.c_int, i32p, i32p]
+        L.or_ploc_tree.restype = C.c_int
         L.or_camera_basis.argtypes = [f32p, f32p, f32p, C.c_float, C.c_int, C.c_int, C.POINTER(Cam)]
         for suf, fp in (("f32", f32p), ("f64", f64p)):
             getattr(L, f"or_phys_prepare_{suf}").argtypes = [f32p, C.c_int, f32p, C.c_void_p]

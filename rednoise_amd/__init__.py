@@ -202,7 +202,8 @@ def _configure(lib: C.CDLL) -> None:
         "rn_trace_primary_compat": ([vp, i, i, f, vp, vp, vp, vp, vp], i),
         "rn_scene_build": ([vp, i, C.POINTER(BvhInfo)], i), "rn_scene_export_bvh": ([vp, vp, vp], i),
         "rn_scene_set_builder": ([vp, i], i),
-        "rn_lbvh_stages": ([vp, i, vp, vp, vp, vp], i), "rn_radix_sort_pairs": ([vp, vp, i, i], i),
+        "rn_lbvh_stages": ([vp, i, vp, vp, vp, vp], i), "rn_ploc_stages": ([vp, i, vp, vp, vp], i),
+        "rn_radix_sort_pairs": ([vp, vp, i, i], i),
         "rn_philox4x32_10": ([i, vp, vp, vp], i),
         "rn_photon_map_trace": ([vp, i, C.c_uint32, C.c_uint32], vp),
         "rn_photon_map_create": ([i, vp, vp, vp, f], vp), "rn_photon_map_free": ([vp], None),
@@ -731,6 +732,19 @@ def lbvh_stages(verts):
     if _load_library().rn_lbvh_stages(_ptr(v), n, _ptr(codes), _ptr(order), _ptr(left), _ptr(right)) != 1:
         raise _err("rn_lbvh_stages")
     return codes, order, left[: n - 1], right[: n - 1]
+
+
+def ploc_stages(verts):
+    """Sorted order and PLOC binary tree of the default builder (left/right:
+    >= 0 internal node, < 0 leaf ~sorted position)."""
+    v = _f32(verts)
+    n = v.size // 9
+    order = np.empty(n, np.int32)
+    left = np.empty(max(n - 1, 1), np.int32)
+    right = np.empty(max(n - 1, 1), np.int32)
+    if _load_library().rn_ploc_stages(_ptr(v), n, _ptr(order), _ptr(left), _ptr(right)) != 1:
+        raise _err("rn_ploc_stages")
+    return order, left[: n - 1], right[: n - 1]
 
 
 def radix_sort_pairs(keys, values, bits=32):

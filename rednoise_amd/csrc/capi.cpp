@@ -1301,6 +1301,19 @@ int rn_lbvh_stages(const float *verts, int n, uint32_t *morton_sorted, int *orde
 	RN_CATCH(0)
 }
 
+int rn_ploc_stages(const float *verts, int n, int *order, int *left, int *right) {
+	RN_TRY
+	t_error.clear();
+	if (!verts || n <= 0 || !order || (n >= 2 && (!left || !right)))
+		throw Error("rn_ploc_stages: bad arguments");
+	Context &c = ctx();
+	std::lock_guard<std::mutex> lk(c.mu);
+	const float *d_v = upload(c.in0, verts, 9 * (size_t)n, c.stream);
+	ploc_stages(d_v, n, order, left, right, c.stream);
+	return 1;
+	RN_CATCH(0)
+}
+
 int rn_radix_sort_pairs(uint32_t *keys, uint32_t *values, int n, int bits) {
 	RN_TRY
 	t_error.clear();

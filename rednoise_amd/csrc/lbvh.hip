@@ -413,12 +413,23 @@ __device__ __forceinline__ float merged_area(float4 alo, float4 ahi, float4 blo,
 	float dx = fmaxf(ahi.x, bhi.x) - fminf(alo.x, blo.x);
 	float dy = fmaxf(ahi.y, bhi.y) - fminf(alo.y, blo.y);
 	float dz = fmaxf(ahi.z, bhi.z) - fminf(alo.z, blo.z);
-	return dx * dy + dy * dz + dz * dx;
+	return fmaf(dz, dx, fmaf(dx, dy, dy * dz)); // fixed evaluation order: the tree is bit-reproducible
 }
 
 // Nearest neighbour of every cluster. Pairs are ordered by (merged area,
-// smaller index, larger index), a strict total order, so the least pair of
-// the round is always mutual and every round merges at least once.
+// parity of the smaller index, smaller index, larger index), a strict total
+// order, so the least pair of the round is always mutual and every round
+// merges at least once. The parity key matters on runs of equal areas (a
+// strip of identical quads): with it every other pair of such a run is
+// mutual, where ordering by index alone leaves only the run's first pair
+// mutual and the run takes a round per merge. Candidates come in increasing
+// j, so along the loop the last two keys increase and a candidate replaces
+// the best on a smaller area, or on an equal one with a smaller parity.
+__device__ __forceinline__ bool ploc_better(float a, int j, float best, int bj, int i) {
+	const int par = (j < i ? j : i) & 1;
+	const int bpar = bj < 0 ? 2 : ((bj < i ? bj : i) & 1);
+	return a < best || (a == best && par < bpar);
+}
 __global__ __launch_bounds__(kBlock) void k_ploc_nn(const uint32_t *pm, const float4 *cbox, int *nn) {
 	__shared__ float4 slo[kBlock + 2 * kPlocR], shi[kBlock + 2 * kPlocR];
 	const int m = (int)*pm;
@@ -445,10 +456,7 @@ __global__ __launch_bounds__(kBlock) void k_ploc_nn(const uint32_t *pm, const fl
 		if (j == i)
 			continue;
 		float a = merged_area(alo, ahi, slo[j - base], shi[j - base]);
-		// candidates come in increasing j: on equal area the earlier one has
-		// the smaller (min, max) index pair, except across i, where j < i
-		// gives (j, i) and j > i gives (i, j): (j, i) < (i, j') as j < i
-		if (a < best) {
+		if (ploc_better(a, j, best, bj, i)) {
 			best = a;
 			bj = j;
 		}
@@ -682,7 +690,7 @@ __global__ __launch_bounds__(kPlocTailBlock) void k_ploc_tail(int n, int cur, in
 				if (j == i)
 					continue;
 				float a = merged_area(alo, ahi, sbox[2 * j], sbox[2 * j + 1]);
-				if (a < best) {
+				if (ploc_better(a, j, best, bj, i)) {
 					best = a;
 					bj = j;
 				}
@@ -1304,6 +1312,17 @@ void lbvh_stages(const float *d_verts, int n, uint32_t *h_codes, int *h_order, i
 	Bvh b;
 	lbvh_front(b, d_verts, n, s);
 	RN_HIP(hipMemcpyAsync(h_codes, b.keys.ptr, (size_t)n * 4, hipMemcpyDeviceToHost, s));
+	RN_HIP(hipMemcpyAsync(h_order, b.vals.ptr, (size_t)n * 4, hipMemcpyDeviceToHost, s));
+	if (n >= 2) {
+		RN_HIP(hipMemcpyAsync(h_left, b.left.ptr, (size_t)(n - 1) * 4, hipMemcpyDeviceToHost, s));
+		RN_HIP(hipMemcpyAsync(h_right, b.right.ptr, (size_t)(n - 1) * 4, hipMemcpyDeviceToHost, s));
+	}
+	RN_HIP(hipStreamSynchronize(s));
+}
+
+void ploc_stages(const float *d_verts, int n, int *h_order, int *h_left, int *h_right, hipStream_t s) {
+	Bvh b;
+	lbvh_front(b, d_verts, n, s, /*karras=*/false);
 	RN_HIP(hipMemcpyAsync(h_order, b.vals.ptr, (size_t)n * 4, hipMemcpyDeviceToHost, s));
 	if (n >= 2) {
 		RN_HIP(hipMemcpyAsync(h_left, b.left.ptr, (size_t)(n - 1) * 4, hipMemcpyDeviceToHost, s));

@@ -69,6 +69,8 @@ void build_bvh(Bvh &bvh, const float *d_verts, int n, int width, hipStream_t s, 
 // Stages exposed for testing (device buffers in, host copies out).
 void lbvh_stages(const float *d_verts, int n, uint32_t *h_codes, int *h_order, int *h_left, int *h_right,
                  hipStream_t s);
+// The PLOC builder's sorted order and binary tree (same layout as lbvh_stages).
+void ploc_stages(const float *d_verts, int n, int *h_order, int *h_left, int *h_right, hipStream_t s);
 void radix_sort_pairs(uint32_t *d_keys, uint32_t *d_vals, uint32_t *d_keys_tmp, uint32_t *d_vals_tmp, int n,
                       int bits, DevBuf &hist, DevBuf &scan_tmp, hipStream_t s);
 // Exclusive prefix sum of n words; tmp holds device_scan_tmp_words(n) words.

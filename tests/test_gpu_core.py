@@ -64,6 +64,43 @@ def test_lbvh_stages_match_oracle(rn, oracle, n, dup):
     assert np.array_equal(right, orr[: n - 1])
 
 
+def strip(q):
+    """q identical unit quads in a row, two triangles each: runs of equal
+    merged areas."""
+    x = np.arange(q, dtype=np.float32)
+    a = np.stack([x, 0 * x, 0 * x], 1)
+    b, c, d = a + [1, 0, 0], a + [1, 1, 0], a + [0, 1, 0]
+    return np.concatenate([np.stack([a, b, c], 1), np.stack([a, c, d], 1)]).astype(np.float32)
+
+
+def grid(g):
+    y, x = np.mgrid[0:g, 0:g].astype(np.float32)
+    a = np.stack([x.ravel(), y.ravel(), 0 * x.ravel()], 1)
+    b, c, d = a + [1, 0, 0], a + [1, 1, 0], a + [0, 1, 0]
+    return np.concatenate([np.stack([a, b, c], 1), np.stack([a, c, d], 1)]).astype(np.float32)
+
+
+@pytest.mark.parametrize("case", ["soup2", "soup3", "soup1000", "soup5000dup", "soup70000", "strip", "grid"])
+def test_ploc_tree_matches_oracle(rn, oracle, case):
+    """The default builder's binary tree, node for node, against the oracle's
+    sequential PLOC; 70000 triangles runs the multi-kernel rounds before the
+    one-work-group tail, the strip and grid have many equal areas."""
+    v = {"soup2": lambda: soup(2, seed=2), "soup3": lambda: soup(3, seed=3), "soup1000": lambda: soup(1000, 1),
+         "soup5000dup": lambda: soup(5000, 5, dup=700), "soup70000": lambda: soup(70000, 7),
+         "strip": lambda: strip(20000), "grid": lambda: grid(120)}[case]()
+    n = len(v)
+    order, left, right = rn.ploc_stages(v)
+    oc = np.empty(n, np.uint32)
+    oracle.lib().or_morton_codes(np.ascontiguousarray(v.reshape(-1)), n, oc)
+    o2 = np.argsort(oc, kind="stable").astype(np.int32)
+    assert np.array_equal(order, o2)
+    ol = np.empty(max(n - 1, 1), np.int32)
+    orr = np.empty(max(n - 1, 1), np.int32)
+    assert oracle.lib().or_ploc_tree(np.ascontiguousarray(v.reshape(-1)), o2, n, ol, orr) > 0
+    assert np.array_equal(left, ol[: n - 1])
+    assert np.array_equal(right, orr[: n - 1])
+
+
 NODE = np.dtype([("p", "<f4", 3), ("e", "u1", 3), ("imask", "u1"), ("child_base", "<u4"), ("tri_base", "<u4"),
                  ("tmask", "<u4"), ("pad", "<u4"), ("qlo", "u1", (3, 8)), ("qhi", "u1", (3, 8))])
 

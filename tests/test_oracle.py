@@ -232,6 +232,95 @@ def test_radix_tree_structure(oracle, n):
     _check_radix_tree(codes, left, right)
 
 
+def _boxes_row(xs):
+    """Triangles in sequence order, the k-th with box [x, x+1] x [0, 1] x [0, 1]:
+    a merged box spanning dx along x has half-area 2 dx + 1."""
+    return np.array([[[x, 0, 0], [x + 1, 1, 0], [x, 0, 1]] for x in xs], np.float32)
+
+
+def _ploc(oracle, v, order=None):
+    n = len(v)
+    order = np.arange(n, dtype=np.int32) if order is None else order
+    left = np.empty(max(n - 1, 1), np.int32)
+    right = np.empty(max(n - 1, 1), np.int32)
+    rounds = oracle.lib().or_ploc_tree(np.ascontiguousarray(v.reshape(-1)), order, n, left, right)
+    return rounds, left[: n - 1].tolist(), right[: n - 1].tolist()
+
+
+# Hand-worked trees: (x positions, rounds, left, right). Ids count down from
+# n - 2 in merge order; the smaller position of a pair is the left child.
+PLOC_CASES = [
+    ([0, 5], 1, [~0], [~1]),
+    # two separate pairs, one round each level
+    ([0, 1.5, 10, 11.5], 2, [2, ~2, ~0], [1, ~3, ~1]),
+    # gaps growing to the right: one merge a round, a caterpillar from the left
+    ([0, 2, 5, 11, 24], 4, [1, 2, 3, ~0], [~4, ~3, ~2, ~1]),
+    # its mirror: the caterpillar grows from the right, owners on the left
+    ([0, 13, 19, 22, 24], 4, [~0, ~1, ~2, ~3], [1, 2, 3, ~4]),
+    # equal gaps: on equal areas the pairs at even positions win, so every
+    # other pair merges (ordering by position alone would merge one a round)
+    ([0, 2, 4, 6, 8, 10], 3, [1, 4, ~4, ~2, ~0], [2, 3, ~5, ~3, ~1]),
+    # duplicates merge first
+    ([0, 0, 10], 2, [1, ~0], [~2, ~1]),
+]
+
+
+@pytest.mark.parametrize("xs,rounds,left,right", PLOC_CASES)
+def test_ploc_tree_known(oracle, xs, rounds, left, right):
+    assert _ploc(oracle, _boxes_row(xs)) == (rounds, left, right)
+
+
+def _check_ploc_tree(n, left, right):
+    """Every internal id 0..n-2 and every leaf is a child exactly once (the
+    root 0 excepted), and children carry larger ids than their parent."""
+    internal = [c for c in left + right if c >= 0]
+    leaves = [~c for c in left + right if c < 0]
+    assert sorted(internal) == list(range(1, n - 1))
+    assert sorted(leaves) == list(range(n))
+    for p in range(n - 1):
+        for c in (left[p], right[p]):
+            assert c < 0 or c > p
+
+
+@pytest.mark.parametrize("n", [2, 3, 200, 3000])
+def test_ploc_tree_structure_and_first_round(oracle, n):
+    rng = np.random.default_rng(n)
+    v = (rng.uniform(-5, 5, (n, 1, 3)) + rng.normal(scale=0.3, size=(n, 3, 3))).astype(np.float32)
+    codes = np.empty(n, np.uint32)
+    oracle.lib().or_morton_codes(np.ascontiguousarray(v.reshape(-1)), n, codes)
+    order = np.argsort(codes, kind="stable").astype(np.int32)
+    rounds, left, right = _ploc(oracle, v, order)
+    assert rounds >= 1
+    _check_ploc_tree(n, left, right)
+    # brute force of the first round: mutual nearest neighbours within 8
+    # positions by merged half-area (float64; random data has no ties)
+    lo = v[order].min(axis=1).astype(np.float64)
+    hi = v[order].max(axis=1).astype(np.float64)
+    nn = []
+    for i in range(n):
+        js = [j for j in range(max(0, i - 8), min(n, i + 9)) if j != i]
+        d = [np.maximum(hi[i], hi[j]) - np.minimum(lo[i], lo[j]) for j in js]
+        nn.append(js[int(np.argmin([e[0] * e[1] + e[1] * e[2] + e[2] * e[0] for e in d]))])
+    pairs = [(i, nn[i]) for i in range(n) if nn[nn[i]] == i and i < nn[i]]
+    first = [(left[n - 2 - k], right[n - 2 - k]) for k in range(len(pairs))]
+    assert first == [(~i, ~j) for i, j in pairs]
+
+
+def test_ploc_rounds_on_a_strip(oracle):
+    """A strip of 4000 identical quads (8000 triangles) in Morton order: the
+    equal areas along it must not serialise the rounds."""
+    q = np.arange(4000, dtype=np.float32)
+    a = np.stack([q, 0 * q, 0 * q], 1)
+    b, c, d = a + [1, 0, 0], a + [1, 1, 0], a + [0, 1, 0]
+    v = np.concatenate([np.stack([a, b, c], 1), np.stack([a, c, d], 1)]).astype(np.float32)
+    n = len(v)
+    codes = np.empty(n, np.uint32)
+    oracle.lib().or_morton_codes(np.ascontiguousarray(v.reshape(-1)), n, codes)
+    rounds, left, right = _ploc(oracle, v, np.argsort(codes, kind="stable").astype(np.int32))
+    _check_ploc_tree(n, left, right)
+    assert 0 < rounds <= 40, rounds
+
+
 # ---------------------------------------------------- physical estimator --
 
 def sphere_inside(levels=3):
