@@ -31,7 +31,10 @@ const char *rn_last_error(void);
 /* Scene from plain arrays: verts = 9*n floats (v0,v1,v2 per triangle),
  * albedo = 3*n (linear, 0..1), emission = 3*n radiance or NULL (no emitters).
  * Triangles with any emission component > 0 form the light set; the light set
- * is fixed at creation. Generalises rn_scene_load_obj (src/capi.cpp:22-34). */
+ * is fixed at creation. Generalises rn_scene_load_obj (src/capi.cpp:22-34).
+ * Every vertex coordinate must be finite: a scene holding a NaN or infinite
+ * one (from any constructor) fails its first build or render with an error
+ * naming the triangle. */
 rn_scene *rn_scene_from_arrays(const float *verts, int n, const float *albedo, const float *emission);
 
 /* Replace per-triangle albedo / emission (3*n each; either may be NULL). The

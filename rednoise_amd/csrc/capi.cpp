@@ -99,6 +99,17 @@ Context &ctx() {
 	return c;
 }
 
+// The BVH builders order and cluster triangles by their boxes and centroids:
+// a NaN or infinite coordinate has no place in that order (the clustering
+// would never merge its triangle), so geometry is checked before it reaches
+// the device.
+void check_finite(const float *verts, int n, const char *who) {
+	for (size_t i = 0; i < 9 * (size_t)n; i++)
+		if (!std::isfinite(verts[i]))
+			throw Error(std::string(who) + ": triangle " + std::to_string(i / 9) +
+			            " has a non-finite vertex coordinate");
+}
+
 template <class T> T *upload(DevBuf &b, const T *h, size_t count, hipStream_t s) {
 	T *d = static_cast<T *>(b.reserve(count * sizeof(T)));
 	if (count)
@@ -139,6 +150,7 @@ struct rn_scene {
 		if (uploaded)
 			return;
 		const int n = h.n;
+		check_finite(h.verts.data(), n, "scene");
 		upload(d_verts, h.verts.data(), h.verts.size(), s);
 		upload(d_albedo, h.albedo.data(), h.albedo.size(), s);
 		upload(d_emission, h.emission.data(), h.emission.size(), s);
@@ -1293,6 +1305,7 @@ int rn_lbvh_stages(const float *verts, int n, uint32_t *morton_sorted, int *orde
 	t_error.clear();
 	if (!verts || n <= 0 || !morton_sorted || !order || (n >= 2 && (!left || !right)))
 		throw Error("rn_lbvh_stages: bad arguments");
+	check_finite(verts, n, "rn_lbvh_stages");
 	Context &c = ctx();
 	std::lock_guard<std::mutex> lk(c.mu);
 	const float *d_v = upload(c.in0, verts, 9 * (size_t)n, c.stream);
@@ -1306,6 +1319,7 @@ int rn_ploc_stages(const float *verts, int n, int *order, int *left, int *right)
 	t_error.clear();
 	if (!verts || n <= 0 || !order || (n >= 2 && (!left || !right)))
 		throw Error("rn_ploc_stages: bad arguments");
+	check_finite(verts, n, "rn_ploc_stages");
 	Context &c = ctx();
 	std::lock_guard<std::mutex> lk(c.mu);
 	const float *d_v = upload(c.in0, verts, 9 * (size_t)n, c.stream);

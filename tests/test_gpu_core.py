@@ -101,6 +101,19 @@ def test_ploc_tree_matches_oracle(rn, oracle, case):
     assert np.array_equal(right, orr[: n - 1])
 
 
+@pytest.mark.parametrize("bad", [np.nan, np.inf, -np.inf])
+def test_non_finite_vertices_are_refused(rn, bad):
+    v = soup(100, seed=9)
+    v[37, 1, 2] = bad
+    with pytest.raises(RuntimeError, match="triangle 37 has a non-finite vertex"):
+        rn.ploc_stages(v)
+    with pytest.raises(RuntimeError, match="triangle 37 has a non-finite vertex"):
+        rn.lbvh_stages(v)
+    sc = rn.Scene.from_arrays(v, np.full((100, 3), 0.5, np.float32), None)
+    with pytest.raises(RuntimeError, match="triangle 37 has a non-finite vertex"):
+        sc.build(8)
+
+
 NODE = np.dtype([("p", "<f4", 3), ("e", "u1", 3), ("imask", "u1"), ("child_base", "<u4"), ("tri_base", "<u4"),
                  ("tmask", "<u4"), ("pad", "<u4"), ("qlo", "u1", (3, 8)), ("qhi", "u1", (3, 8))])
 
