@@ -355,6 +355,24 @@ def test_bench_scene_closest_and_any_vs_brute(rn, oracle):
     _surface_rays_vs_brute(rn, oracle, scenes.config1_scene(0), 384, 128, 22)
 
 
+def test_tie_heavy_scene_closest_and_any_vs_brute(rn, oracle):
+    """Runs of equal merged areas shape the clustering tree: a 150 x 150 grid
+    of identical quads (a floor at y = 100) and a strip of 2,000 identical
+    quads (at y = 300) inside the Cornell box, against brute force."""
+    from rednoise_amd import scenes
+    box = scenes.cornell_box()
+    g = grid(150) * np.float32(3.0)
+    g = np.stack([50 + g[..., 0], np.full_like(g[..., 0], 100.0), 50 + g[..., 1]], -1)
+    st = strip(2000) * np.float32(0.2)
+    st = np.stack([50 + st[..., 0], np.full_like(st[..., 0], 300.0), 250 + st[..., 1]], -1)
+    extra = np.concatenate([g, st]).astype(np.float32)
+    sd = type(box)(**{**box.__dict__,
+                      "verts": np.concatenate([box.verts, extra]).astype(np.float32),
+                      "albedo": np.concatenate([box.albedo, np.full((len(extra), 3), 0.5, np.float32)]),
+                      "emission": np.concatenate([box.emission, np.zeros((len(extra), 3), np.float32)])})
+    _surface_rays_vs_brute(rn, oracle, sd, 1000, 300, 23)
+
+
 def _edge_band(u, v, eps=1e-6):
     return np.minimum(np.minimum(u, v), 1.0 - u - v) < eps
 
