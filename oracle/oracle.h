@@ -141,8 +141,8 @@ void or_radix_tree(const uint32_t *sorted_codes, int n, int *left, int *right);
  * arrays (internal ids 0 .. n-2 with the root at 0, leaves ~position).
  * Returns the number of clustering rounds (-1: a round merged nothing).
  * [pinned: hand-worked trees (two pairs, a caterpillar and its mirror,
- *  duplicates); structural invariants; brute-force mutual-pair check of the
- *  first round.] */
+ *  equal gaps, duplicates); structural invariants; brute-force mutual-pair
+ *  check of the first round.] */
 int or_ploc_tree(const float *verts, const int *order, int n, int *left, int *right);
 
 /* -------------------------------------------------- physical estimator -- */
